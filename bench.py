@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the vortex-filament velocity / RK4 path on one MI355X.
+
+Workload: the Barnes-Hut-scale periodic tangle (box 0.2 cm, periodic-26
+images, 64 perturbed box-spanning lines + log-uniform rings at 1e-4 cm
+spacing, ~1.0M points, seed 1002) with velocities from the direct
+Biot-Savart sum + LIA + mutual friction.
+
+A step is one RK4 step (ov_gpu_rk4_run, device-resident): 4 velocity
+evaluations + geometry + stage updates, in the mixed form (FP32 far field,
+FP64 near field / LIA / RK).  The timed region is device time (HIP events on
+the library's stream); inputs are already in HBM.
+
+value = segment-point interactions per second of the timed RK4 steps.
+Also reported: RK4 steps/s, the plain FP64 form's rate on one evaluation of
+the same positions, the mixed form's error against it, the roofline of the
+dominant kernel (k_direct) and a CPU baseline: the reference's own OpenMP
+update_velocity (oracle/_ref, or the FP64 restatement where that is absent)
+on a seeded sample of targets of the same tangle, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_INTERACTION = 43          # 40 add/mul (FMA = 2) + 2 sqrt + 1 div
+PEAK_FP32 = 157.3e12               # MI355X vector FP32, FLOP/s
+PEAK_FP64 = 78.6e12                # MI355X vector FP64, FLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def err_stats(x, y):
+    d = np.linalg.norm(x - y, axis=1)
+    n = np.linalg.norm(y, axis=1)
+    r = d / np.maximum(n, 1e-300)
+    return {"max_rel": float(r.max()), "rms_rel": float(np.sqrt(np.mean(r * r))),
+            "max_abs_over_rms": float(d.max() / np.sqrt(np.mean(n * n)))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--points", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=1002)
+    ap.add_argument("--dt", type=float, default=1e-6)
+    ap.add_argument("--near-radius", type=float, default=0.005)
+    ap.add_argument("--fp64", type=int, default=1, help="FP64 evaluations for the comparison (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="targets of the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "sharding of this path is described in DESIGN.md but not built", flush=True)
+        sys.exit(2)
+
+    from openvort_amd import tangle as T
+    from openvort_amd.gpu import Context
+
+    t0 = time.time()
+    tg = T.bh_scale(seed=args.seed, n_points=args.points)
+    n = len(tg.active)
+    log(f"workload: {n} points, generated in {time.time() - t0:.1f}s")
+    common = dict(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, hyperfriction=0)
+    p_mx = T.default_params(precision=T.PREC_MIXED, near_radius=args.near_radius, **common)
+    p_64 = T.default_params(precision=T.PREC_FP64, **common)
+
+    ctx = Context(0)
+    # load the code objects on a tiny case (not a step of the workload)
+    ctx.velocities(T.ring(64), 0.0, p_mx)
+
+    for w in range(args.warmup):
+        x = tg.copy()
+        ctx.rk4_run(x, 0.0, args.dt, 1, p_mx)
+        log(f"warmup step {w}: {ctx.last_device_ms():.1f} ms")
+
+    # ---- timed: K RK4 steps, device-resident -------------------------------
+    x = tg.copy()
+    ctx.reset_stats()
+    wall0 = time.time()
+    ctx.rk4_run(x, 0.0, args.dt, args.steps, p_mx, profile=True)
+    wall = time.time() - wall0
+    dev_ms = ctx.last_device_ms()
+    I_eval = ctx.last_interactions()
+    n_launch, kd_ms = ctx.kernel_stats("k_direct")
+    ms_per_step = dev_ms / args.steps
+    value = 4 * I_eval * args.steps / (dev_ms / 1e3)
+    log(f"timed: {args.steps} rk4 steps, device {dev_ms:.1f} ms, wall {wall:.1f} s, "
+        f"{value:.4g} int/s, k_direct {n_launch} launches {kd_ms:.1f} ms")
+    kd_avg_s = kd_ms / max(n_launch, 1) / 1e3
+    achieved = FLOP_PER_INTERACTION * I_eval / kd_avg_s
+    kernels = {}
+    for k in ("k_geometry", "k_cont", "k_scan_blocks", "k_records", "k_src_tiles", "k_tgt_tiles",
+              "k_direct", "k_reduce", "k_epilogue", "k_stage", "k_combine"):
+        nl, ms = ctx.kernel_stats(k)
+        if nl:
+            kernels[k] = {"launches": nl, "avg_ms": round(ms / nl, 4)}
+
+    # ---- one mixed and one FP64 evaluation of the same positions ------------
+    out = {}
+    if args.fp64 > 0:
+        a = tg.copy()
+        ctx.reset_stats()
+        ctx.velocities(a, 0.0, p_mx, profile=True)
+        mx_ms = ctx.last_device_ms()
+        b = tg.copy()
+        ctx.reset_stats()
+        ctx.velocities(b, 0.0, p_64, profile=True)
+        f64_ms = ctx.last_device_ms()
+        nl64, kd64 = ctx.kernel_stats("k_direct")
+        act = tg.active
+        out["fp64_eval_ms"] = round(f64_ms, 2)
+        out["mixed_eval_ms"] = round(mx_ms, 2)
+        out["fp64_interactions_per_s"] = I_eval / (f64_ms / 1e3)
+        out["mixed_interactions_per_s_eval"] = I_eval / (mx_ms / 1e3)
+        out["mixed_vs_fp64"] = round(f64_ms / mx_ms, 3)
+        out["err_vels_mixed_vs_fp64"] = err_stats(a.vels[act], b.vels[act])
+        out["err_vs_mixed_vs_fp64"] = err_stats(a.vs[act], b.vs[act])
+        out["fp64_roofline"] = {
+            "bound": "valu", "achieved": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / 1e12, 3),
+            "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+            "frac": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / PEAK_FP64, 4)}
+        log(f"fp64 eval {f64_ms:.1f} ms, mixed eval {mx_ms:.1f} ms, "
+            f"err {out['err_vels_mixed_vs_fp64']}")
+
+    # ---- CPU baseline: the reference's update_velocity on sampled targets ---
+    cpu = None
+    if args.cpu_sample > 0:
+        from oracle.oracle import Oracle, Reference
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        rng = np.random.default_rng(2002)
+        idx = np.sort(rng.choice(tg.active, size=args.cpu_sample, replace=False)).astype(np.int32)
+        c = tg.copy()
+        Oracle(threads).update_tangents_normals(c)
+        if Reference.available():
+            kind, eng = "reference", Reference(threads)
+        else:
+            kind, eng = "port", Oracle(threads)
+        c0 = time.time()
+        vs_c, vel_c = eng.velocities_sample(c, 0.0, p_64, idx)
+        cpu_s = time.time() - c0
+        n_seg = n
+        inter = args.cpu_sample * (27 * n_seg - 2)
+        cpu = {"value": inter / cpu_s, "unit": "seg-interactions/s", "cores": threads, "kind": kind,
+               "sample": f"{args.cpu_sample} seeded targets of the same tangle, FP64 direct sum + "
+                         f"26 images ({inter:.3g} interactions, {cpu_s:.1f} s)"}
+        log(f"cpu baseline ({kind}, {threads} threads): {cpu['value']:.4g} int/s in {cpu_s:.1f}s")
+        if args.fp64 > 0:
+            out["err_vels_fp64_vs_cpu_sample"] = err_stats(b.vels[idx], vel_c)
+            out["err_vels_mixed_vs_cpu_sample"] = err_stats(a.vels[idx], vel_c)
+
+    ctx.close()
+    res = {
+        "metric": "Biot-Savart seg-interactions/s (mixed FP32/FP64 direct sum, RK4 steps)",
+        "value": value,
+        "unit": "seg-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "periodic tangle, box 0.2 cm, 64 perturbed lines + log-uniform "
+                               "rings, spacing 1e-4 cm, seed %d" % args.seed,
+                   "points": n, "images": 26, "method": "direct", "precision": "mixed",
+                   "near_radius_cm": args.near_radius, "dt_s": args.dt,
+                   "interactions_per_eval": I_eval},
+        "rk4_steps_per_s": 1e3 / ms_per_step,
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4), "traffic": None,
+                     "kernel": "k_direct", "avg_launch_ms": round(kd_avg_s * 1e3, 2),
+                     "flop_per_interaction": FLOP_PER_INTERACTION},
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    res.update(out)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * openvort_gpu.h — C ABI of the MI355X vortex-filament velocity / RK4 library.
+ *
+ * Drop-in boundary for OpenVort's hot path.  The host keeps its own
+ * `struct tangle_state` (reference include/tangle.h:82-106), configuration,
+ * reconnection and remeshing; it hands the tangle to these entry points in
+ * place of
+ *   update_velocities()        reference include/tangle.h:128, src/tangle.c:686-702
+ *   update_tangle()            reference include/tangle.h:160-164
+ *   update_tangents_normals()  reference include/tangle.h:120, src/tangle.c:723-727
+ *   rk4_step()                 reference include/vortex_dynamics.h:49,
+ *                              src/vortex_dynamics.c:58-121
+ *
+ * Every type below is layout-identical to its reference counterpart, so a
+ * reference host passes `(ov_tangle_state *)&tangle` without copying.  The
+ * physics globals of reference src/vortex_constants.c:24-95 and the flow
+ * descriptors of include/external_velocity.h:80-85 travel in `ov_params`.
+ *
+ * Conventions (all entry points):
+ *   - return OV_OK (0) or a negative OV_E_* code; never abort; the message of
+ *     the last failure on a context is ov_last_error(ctx);
+ *   - host arrays are caller-owned and re-read on every call (the reference
+ *     reallocates them, src/tangle.c:114-150), nothing keeps host pointers;
+ *   - calls are synchronous on return; one context per host thread;
+ *   - a non-finite velocity or position (the reference would trap with
+ *     SIGFPE, src/main.c:56) is reported as OV_E_NONFINITE after the outputs
+ *     have been written.
+ */
+#ifndef OPENVORT_GPU_H
+#define OPENVORT_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- tangle types, layout-identical to the reference -------------------- */
+
+typedef struct { double p[3]; } ov_vec3d;              /* vec3_maths.h:23-25 */
+typedef struct { int forward, reverse; } ov_neighbour;  /* tangle.h:41-44 */
+typedef struct { int status, pin_wall; } ov_node_status;/* tangle.h:36-39 */
+
+enum { OV_EMPTY = -1, OV_FREE = 0, OV_PINNED = 1, OV_PINNED_SLIP = 2 }; /* tangle.h:28-33 */
+enum { OV_WALL_OPEN = 0, OV_WALL_PERIODIC = 1, OV_WALL_MIRROR = 2 };    /* vec3_maths.h:38 */
+enum { OV_X_L = 0, OV_X_H, OV_Y_L, OV_Y_H, OV_Z_L, OV_Z_H };            /* vec3_maths.h:46-51 */
+
+typedef struct {                                        /* vec3_maths.h:61-68 */
+  ov_vec3d bottom_left_back;
+  ov_vec3d top_right_front;
+  int wall[6];
+} ov_domain_box;
+
+typedef struct { int shift[3]; int reflect; } ov_image_tangle;   /* tangle.h:50-53 */
+typedef struct { const ov_image_tangle *images; int n; } ov_boundary_images; /* tangle.h:55-58 */
+
+typedef struct {                                        /* tangle.h:82-106 */
+  ov_vec3d *vnodes;
+  ov_vec3d *vnodes_new;
+  ov_vec3d *vels;
+  ov_vec3d *vs;
+  ov_vec3d *tangents;
+  ov_vec3d *normals;
+  double *dxi;
+  int *recalculate;
+  ov_neighbour *connections;
+  ov_node_status *status;
+  ov_domain_box box;
+  ov_boundary_images bimg;
+  int N;
+  int next_free;
+  int total_free;
+} ov_tangle_state;
+
+/* ---- parameters: the reference globals + flow descriptors --------------- */
+
+/* Imposed flows (external_velocity.c:37-50, 176-188).  Only the flows the
+ * workloads use run on the device; any other type is OV_E_UNSUPPORTED. */
+enum { OV_FLOW_NONE = 0,     /* "no flow"  external_velocity.c:176-182 */
+       OV_FLOW_SIMPLE = 1 }; /* "simple"   external_velocity.c:184-188 */
+typedef struct { int type; double v[3]; } ov_flow;
+
+enum { OV_PREC_FP64 = 0,     /* every pair term in FP64 */
+       OV_PREC_MIXED = 1 };  /* FP32 far field, FP64 near field, LIA, RK */
+enum { OV_RK_REFERENCE = 0,  /* k4 at x + dt/2 k3, vortex_dynamics.c:89 */
+       OV_RK_CLASSIC = 1 };  /* k4 at x + dt k3 */
+enum { OV_MP_REFERENCE = 0,  /* octree.c:303-328 as written */
+       OV_MP_DIPOLE = 1 };   /* true first-order (dipole) expansion */
+
+typedef struct {
+  double kappa;               /* KAPPA                       vortex_constants.c:28 */
+  double vortex_width;        /* VORTEX_WIDTH (a0)           vortex_constants.c:26 */
+  double alpha, alpha_p;      /* global_alpha, _alpha_p      vortex_constants.c:34-35 */
+  double dl_max;              /* global_dl_max               vortex_constants.c:44 */
+  int LIA_only;               /* global_LIA_only             vortex_constants.c:31 */
+  int use_mutual_friction;    /* global_use_mutual_friction  vortex_constants.c:53 */
+  int hyperfriction;          /* global_hyperfriction        vortex_constants.c:55 */
+  int use_BH;                 /* global_use_BH               vortex_constants.c:91 */
+  double max_curvature_scale; /* global_max_curvature_scale  vortex_constants.c:56 */
+  double hyperalpha;          /* global_hyperalpha           vortex_constants.c:57 */
+  double BH_resolution;       /* global_BH_resolution (theta) vortex_constants.c:93 */
+  double BH_grain;            /* global_BH_grain             vortex_constants.c:95 */
+  int BH_quadtree;            /* global_BH_quadtree          vortex_constants.c:92 */
+  int precision;              /* OV_PREC_*   (no reference counterpart) */
+  int rk_variant;             /* OV_RK_*     (no reference counterpart) */
+  int multipole;              /* OV_MP_*     (no reference counterpart) */
+  double near_radius;         /* mixed: FP64 below this box distance, <=0: default */
+  ov_flow vn, vs, vb;         /* vn_conf, vs_conf, vb_conf   external_velocity.c:76-78 */
+} ov_params;
+
+/* Error codes. */
+enum {
+  OV_OK = 0,
+  OV_E_INVALID = -1,      /* malformed tangle or arguments */
+  OV_E_UNSUPPORTED = -2,  /* pinned points, walls with pinning, unknown flows */
+  OV_E_NONFINITE = -3,    /* a result is NaN/Inf (outputs still written) */
+  OV_E_DEVICE = -4,       /* HIP runtime failure */
+  OV_E_NOMEM = -5         /* device or host allocation failed */
+};
+
+/* Flags of ov_gpu_velocities / ov_gpu_rk4_step / ov_gpu_rk4_run. */
+enum {
+  OV_UPDATE_GEOMETRY = 1,        /* recompute dxi/tangents/normals first (update_tangle) */
+  OV_USE_CALLER_K1 = 2,          /* rk4: k1 = tangle->vels as given (rk4_step alone, G16) */
+  OV_ENFORCE_BOUNDARIES = 4,     /* rk4_run: periodic wrap after each step (tangle.c:796-819) */
+  OV_PROFILE = 8                 /* time each kernel with HIP events (ov_ctx_kernel_stats) */
+};
+
+typedef struct ov_ctx ov_ctx;
+
+/* Reference defaults of vortex_constants.c:24-95 and the no-flow configs
+ * (external_velocity.c:80-83); precision FP64, reference RK and multipole. */
+void ov_params_default(ov_params *p);
+
+/* Context: device buffers, stream and events for one host thread. */
+ov_ctx *ov_ctx_create(int device);
+void ov_ctx_destroy(ov_ctx *ctx);
+const char *ov_last_error(const ov_ctx *ctx);
+/* hipStream_t the context launches on (for callers that order their own work). */
+void *ov_ctx_stream(ov_ctx *ctx);
+
+/* update_tangents_normals (tangle.h:120): writes dxi, tangents, normals. */
+int ov_gpu_update_tangents_normals(ov_ctx *ctx, ov_tangle_state *tangle);
+
+/* update_velocities (tangle.h:128) at time t: writes vs[] and vels[].
+ * With OV_UPDATE_GEOMETRY it is update_tangle (tangle.h:160-164) and also
+ * writes dxi/tangents/normals; otherwise it reads the caller's tangents and
+ * normals (the reference precondition, tangle.h:124). */
+int ov_gpu_velocities(ov_ctx *ctx, ov_tangle_state *tangle, double t,
+                      const ov_params *params, int flags);
+
+/* update_tangle(t) followed by rk4_step(t, dt) (main.c:112-113): writes the
+ * new vnodes, vels = k1 at t, vs at t and the geometry at t.  With
+ * OV_USE_CALLER_K1 the caller's vels are k1 (rk4_step alone). */
+int ov_gpu_rk4_step(ov_ctx *ctx, ov_tangle_state *tangle, double t, double dt,
+                    const ov_params *params, int flags);
+
+/* nsteps of ov_gpu_rk4_step with the tangle resident on the device between
+ * steps (the host loop of main.c:96-131 without reconnection or remeshing;
+ * with OV_ENFORCE_BOUNDARIES also its periodic wrap, main.c:117). */
+int ov_gpu_rk4_run(ov_ctx *ctx, ov_tangle_state *tangle, double t0, double dt,
+                   int nsteps, const ov_params *params, int flags);
+
+/* Per-kernel timings gathered while OV_PROFILE was set; kernel names are the
+ * device function names.  Returns the number of launches, 0 if none. */
+long ov_ctx_kernel_stats(ov_ctx *ctx, const char *kernel, double *total_ms);
+void ov_ctx_reset_stats(ov_ctx *ctx);
+
+/* Segment-point interactions of the last velocity evaluation (the
+ * direct-sum count N_tgt (N_seg - 2) + n_img N_tgt N_seg). */
+double ov_ctx_last_interactions(const ov_ctx *ctx);
+
+/* Device time of the last velocities / rk4 call in milliseconds: HIP events
+ * on the context stream around the kernels, host<->device copies excluded. */
+double ov_ctx_last_device_ms(const ov_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENVORT_GPU_H */

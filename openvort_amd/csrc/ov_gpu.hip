@@ -1,0 +1,707 @@
+// ov_gpu.hip — host side of libopenvort_gpu: the C ABI of
+// include/openvort_gpu.h, device buffers, stream ordering and launches.
+//
+// One call = validate the caller's tangle, renumber its active points in
+// stream order (loop by loop along forward links), upload SoA FP64 planes,
+// run the kernels of ov_kernels.hip on the context's stream, scatter the
+// results back into the caller's AoS arrays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/openvort_gpu.h"
+#include "ov_device.h"
+#include "ov_kernels.hip"
+
+using namespace ov;
+
+namespace {
+
+struct KStat {
+  long launches = 0;
+  double ms = 0;
+};
+
+struct Pending {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ov_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char err[512] = {0};
+
+  // stream-order maps of the current call
+  int n = 0;              // active points
+  long cap = 0;           // plane stride of the SoA buffers
+  std::vector<int> act;   // stream index -> caller index
+  std::vector<int> fwd_s, rev_s;
+  BoxDev box{};
+  std::vector<ImgDev> imgs;
+  PhysDev ph{};
+  int mixed = 0;
+  double rnear = 0.005;
+  int wall_periodic_mask = 0;
+  bool lia_only = false;
+
+  // device buffers (planes of `cap` doubles)
+  double *X0 = nullptr, *Xs = nullptr, *K[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *T = nullptr, *Nn = nullptr, *dxi = nullptr, *lia = nullptr, *bs = nullptr;
+  double *vs = nullptr, *vel_tmp = nullptr;
+  double *T1 = nullptr, *N1 = nullptr, *dxi1 = nullptr, *vs1 = nullptr;
+  int *fwd = nullptr, *rev = nullptr, *flag = nullptr;
+  int *brk = nullptr, *rank = nullptr, *bsum = nullptr, *counts = nullptr;
+  TgtTile* tgt = nullptr;
+  SrcTile* src = nullptr;
+  FarRec* far = nullptr;
+  NearRec* near = nullptr;
+  ImgDev* img = nullptr;
+  double* partial = nullptr;
+  size_t partial_bytes = 0;
+  long buf_cap = 0;
+
+  // pinned host staging
+  double* hst = nullptr;
+  size_t hst_bytes = 0;
+
+  // profiling
+  bool profile = false;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_pool;
+  std::map<std::string, KStat> stats;
+  double last_interactions = 0;
+  hipEvent_t t_begin = nullptr, t_end = nullptr;
+  double last_device_ms = 0;
+};
+
+namespace {
+
+int fail(ov_ctx* c, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(c->err, sizeof c->err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ctx, OV_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));   \
+  } while (0)
+
+hipEvent_t get_event(ov_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Launch on the context stream, bracketed by HIP events when profiling.
+template <typename K, typename... Args>
+int launch(ov_ctx* ctx, const char* name, K kernel, dim3 grid, dim3 block, Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (ctx->profile) {
+    a = get_event(ctx);
+    b = get_event(ctx);
+    if (a) (void)hipEventRecord(a, ctx->stream);
+  }
+  hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, OV_E_DEVICE, "launch %s: %s", name, hipGetErrorString(e));
+  if (ctx->profile && a && b) {
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({name, a, b});
+  }
+  return OV_OK;
+}
+
+void collect_profile(ov_ctx* ctx) {
+  for (auto& p : ctx->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      KStat& s = ctx->stats[p.name];
+      s.launches += 1;
+      s.ms += ms;
+    }
+    ctx->ev_pool.push_back(p.a);
+    ctx->ev_pool.push_back(p.b);
+  }
+  ctx->pending.clear();
+}
+
+int dalloc(ov_ctx* ctx, void** p, size_t bytes) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(ctx, OV_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return OV_OK;
+}
+
+// Grow every per-point buffer to hold `need` points (x2 growth).
+int ensure_capacity(ov_ctx* ctx, long need) {
+  if (need <= ctx->buf_cap) return OV_OK;
+  long cap = std::max<long>(need, 2 * ctx->buf_cap);
+  cap = ((cap + TILE - 1) / TILE) * TILE;
+  size_t v3 = sizeof(double) * 3 * (size_t)cap;
+  size_t v1 = sizeof(double) * (size_t)cap;
+  int rc = 0;
+  double** v3b[] = {&ctx->X0, &ctx->Xs, &ctx->K[0], &ctx->K[1], &ctx->K[2], &ctx->K[3],
+                    &ctx->T,  &ctx->Nn, &ctx->lia,  &ctx->bs,   &ctx->vs,   &ctx->vel_tmp,
+                    &ctx->T1, &ctx->N1, &ctx->vs1};
+  for (double** b : v3b)
+    if ((rc = dalloc(ctx, (void**)b, v3))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->dxi, v1))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->dxi1, v1))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->fwd, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->rev, sizeof(int) * cap))) return rc;
+  // record stream: at most two records per point, padded to whole tiles
+  const long max_rec = 2 * cap + TILE;
+  if ((rc = dalloc(ctx, (void**)&ctx->brk, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->rank, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->bsum, sizeof(int) * (cap / 256 + 1)))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->counts, sizeof(int) * 4))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->tgt, sizeof(TgtTile) * (cap / TILE)))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->src, sizeof(SrcTile) * (max_rec / TILE)))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->far, sizeof(FarRec) * max_rec))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->near, sizeof(NearRec) * max_rec))) return rc;
+  ctx->buf_cap = cap;
+  return OV_OK;
+}
+
+int ensure_host(ov_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->hst_bytes) return OV_OK;
+  if (ctx->hst) hipHostFree(ctx->hst);
+  ctx->hst = nullptr;
+  size_t b = std::max(bytes, 2 * ctx->hst_bytes);
+  hipError_t e = hipHostMalloc((void**)&ctx->hst, b, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ctx->hst_bytes = 0;
+    return fail(ctx, OV_E_NOMEM, "hipHostMalloc(%zu): %s", b, hipGetErrorString(e));
+  }
+  ctx->hst_bytes = b;
+  return OV_OK;
+}
+
+// Validate the tangle the way the reference's walks require it and build the
+// stream order.  EMPTY points must have no forward link (the reference sums
+// every point whose forward link is set, tangle.c:612-615, and every EMPTY
+// point it creates has none, tangle.c:94-99, 1063-1067).
+int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
+  if (!t || !p) return fail(ctx, OV_E_INVALID, "null tangle or params");
+  if (t->N < 0) return fail(ctx, OV_E_INVALID, "negative N");
+  if (t->N > 0 && (!t->vnodes || !t->connections || !t->status))
+    return fail(ctx, OV_E_INVALID, "null tangle arrays");
+  if (p->vs.type < OV_FLOW_NONE || p->vs.type > OV_FLOW_SIMPLE || p->vn.type < OV_FLOW_NONE ||
+      p->vn.type > OV_FLOW_SIMPLE)
+    return fail(ctx, OV_E_UNSUPPORTED, "only the 'no flow' and 'simple' flows run on the device");
+  if (p->use_BH) return fail(ctx, OV_E_UNSUPPORTED, "octree path not available in this build");
+  if (t->bimg.n < 0 || t->bimg.n > MAX_IMAGES - 1 || (t->bimg.n > 0 && !t->bimg.images))
+    return fail(ctx, OV_E_INVALID, "bad image list (n=%d)", t->bimg.n);
+  const int N = t->N;
+  std::vector<int> cid(N, -1);
+  ctx->act.clear();
+  for (int k = 0; k < N; ++k) {
+    int st = t->status[k].status;
+    if (st == OV_EMPTY) {
+      if (t->connections[k].forward != -1)
+        return fail(ctx, OV_E_INVALID, "EMPTY point %d has a forward link", k);
+      continue;
+    }
+    if (st == OV_PINNED || st == OV_PINNED_SLIP)
+      return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are not supported", k);
+    if (st != OV_FREE) return fail(ctx, OV_E_INVALID, "bad status %d at %d", st, k);
+    int f = t->connections[k].forward, r = t->connections[k].reverse;
+    if (f < 0 || f >= N || r < 0 || r >= N)
+      return fail(ctx, OV_E_INVALID, "free point %d has links %d/%d", k, f, r);
+    for (int nb : {f, r}) {
+      int sn = t->status[nb].status;
+      if (sn == OV_PINNED || sn == OV_PINNED_SLIP)
+        return fail(ctx, OV_E_UNSUPPORTED, "point %d neighbours pinned point %d", k, nb);
+      if (sn != OV_FREE) return fail(ctx, OV_E_INVALID, "point %d links to empty point %d", k, nb);
+    }
+    cid[k] = (int)ctx->act.size();
+    ctx->act.push_back(k);
+  }
+  const int n = (int)ctx->act.size();
+  // stream order: loops in order of their lowest caller index, each walked forward
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<char> seen(n, 0);
+  for (int i = 0; i < n; ++i) {
+    int j = i;
+    while (!seen[j]) {
+      seen[j] = 1;
+      order.push_back(j);
+      j = cid[t->connections[ctx->act[j]].forward];
+    }
+  }
+  std::vector<int> pos(n);
+  for (int q = 0; q < n; ++q) pos[order[q]] = q;
+  std::vector<int> act_s(n);
+  ctx->fwd_s.assign(n, 0);
+  ctx->rev_s.assign(n, 0);
+  for (int q = 0; q < n; ++q) {
+    int k = ctx->act[order[q]];
+    act_s[q] = k;
+    ctx->fwd_s[q] = pos[cid[t->connections[k].forward]];
+    ctx->rev_s[q] = pos[cid[t->connections[k].reverse]];
+  }
+  ctx->act.swap(act_s);
+  ctx->n = n;
+
+  // box and images
+  for (int i = 0; i < 3; ++i) {
+    ctx->box.lo[i] = t->box.bottom_left_back.p[i];
+    ctx->box.hi[i] = t->box.top_right_front.p[i];
+    ctx->box.L[i] = ctx->box.hi[i] - ctx->box.lo[i];
+    ctx->box.periodic[i] = t->box.wall[2 * i] == OV_WALL_PERIODIC;
+  }
+  ctx->wall_periodic_mask = 0;
+  for (int f = 0; f < 6; ++f)
+    if (t->box.wall[f] == OV_WALL_PERIODIC) ctx->wall_periodic_mask |= 1 << f;
+  ctx->imgs.clear();
+  ImgDev prim{};
+  prim.axis = -1;
+  prim.primary = 1;
+  ctx->imgs.push_back(prim);
+  for (int j = 0; j < t->bimg.n; ++j) {
+    const ov_image_tangle& it = t->bimg.images[j];
+    if (it.reflect < -1 || it.reflect > 5)
+      return fail(ctx, OV_E_INVALID, "image %d: bad reflect %d", j, it.reflect);
+    ImgDev d{};
+    for (int i = 0; i < 3; ++i) d.shift[i] = ctx->box.L[i] * it.shift[i];
+    d.axis = -1;
+    if (it.reflect >= 0) {
+      int ax = it.reflect / 2;
+      double wall = (it.reflect & 1) ? ctx->box.hi[ax] : ctx->box.lo[ax];
+      d.axis = ax;
+      d.wall2 = 2 * wall;
+    }
+    d.primary = 0;
+    ctx->imgs.push_back(d);
+  }
+
+  // physics
+  PhysDev& ph = ctx->ph;
+  ph.kappa = p->kappa;
+  ph.a0 = p->vortex_width;
+  ph.alpha = p->alpha;
+  ph.alpha_p = p->alpha_p;
+  ph.dl_max = p->dl_max;
+  ph.max_curv = p->max_curvature_scale;
+  ph.hyperalpha = p->hyperalpha;
+  for (int i = 0; i < 3; ++i) {
+    ph.vs_ext[i] = p->vs.type == OV_FLOW_SIMPLE ? p->vs.v[i] : 0.0;
+    ph.vn_ext[i] = p->vn.type == OV_FLOW_SIMPLE ? p->vn.v[i] : 0.0;
+  }
+  ph.lia_only = p->LIA_only;
+  ph.use_mf = p->use_mutual_friction;
+  ph.hyperfriction = p->hyperfriction;
+  ctx->lia_only = p->LIA_only != 0;
+  ctx->mixed = p->precision == OV_PREC_MIXED;
+  ctx->rnear = p->near_radius > 0 ? p->near_radius : 0.005;
+  ctx->profile = false;
+
+  int rc = ensure_capacity(ctx, std::max(n, 1));
+  if (rc) return rc;
+  ctx->cap = ctx->buf_cap;
+  if (ctx->imgs.size() > 0) {
+    if (!ctx->img && (rc = dalloc(ctx, (void**)&ctx->img, sizeof(ImgDev) * MAX_IMAGES))) return rc;
+  }
+  return OV_OK;
+}
+
+// caller AoS vec3 array -> stream-order SoA planes on the device
+int upload_vec3(ov_ctx* ctx, const ov_vec3d* src, double* dst) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  int rc = ensure_host(ctx, sizeof(double) * 3 * (size_t)cap);
+  if (rc) return rc;
+  double* h = ctx->hst;
+  for (int q = 0; q < n; ++q) {
+    const double* v = src[ctx->act[q]].p;
+    h[q] = v[0];
+    h[cap + q] = v[1];
+    h[2 * cap + q] = v[2];
+  }
+  HIPCHK(hipMemcpyAsync(dst, h, sizeof(double) * 3 * (size_t)cap, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return OV_OK;
+}
+
+int download_vec3(ov_ctx* ctx, const double* src, ov_vec3d* dst) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  int rc = ensure_host(ctx, sizeof(double) * 3 * (size_t)cap);
+  if (rc) return rc;
+  double* h = ctx->hst;
+  HIPCHK(hipMemcpyAsync(h, src, sizeof(double) * 3 * (size_t)cap, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < n; ++q) {
+    double* v = dst[ctx->act[q]].p;
+    v[0] = h[q];
+    v[1] = h[cap + q];
+    v[2] = h[2 * cap + q];
+  }
+  return OV_OK;
+}
+
+int download_scalar(ov_ctx* ctx, const double* src, double* dst) {
+  const int n = ctx->n;
+  int rc = ensure_host(ctx, sizeof(double) * (size_t)ctx->cap);
+  if (rc) return rc;
+  double* h = ctx->hst;
+  HIPCHK(hipMemcpyAsync(h, src, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < n; ++q) dst[ctx->act[q]] = h[q];
+  return OV_OK;
+}
+
+int upload_topology(ov_ctx* ctx) {
+  const int n = ctx->n;
+  HIPCHK(hipMemcpyAsync(ctx->fwd, ctx->fwd_s.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->rev, ctx->rev_s.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->img, ctx->imgs.data(), sizeof(ImgDev) * ctx->imgs.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->flag, 0, sizeof(int), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return OV_OK;
+}
+
+inline dim3 grid1(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// One velocity evaluation at positions X: geometry (or the caller's s', s''
+// already in Tg/Ng), the direct sum, the epilogue.
+int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, double* dxi,
+             double* vs, double* vel) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  int rc;
+  if (n == 0) return OV_OK;
+  if ((rc = launch(ctx, "k_geometry", k_geometry, grid1(n), dim3(256), X, (const int*)ctx->fwd,
+                   (const int*)ctx->rev, n, cap, ctx->box, ctx->ph, geom ? 1 : 0, Tg, Ng, dxi,
+                   ctx->lia)))
+    return rc;
+  int have_bs = 0;
+  if (!ctx->lia_only) {
+    const int n_tgt = (n + TILE - 1) / TILE;
+    const int nb = (n + 255) / 256;
+    const int max_src = (2 * n + TILE - 1) / TILE;
+    const int n_img = (int)ctx->imgs.size();
+    if ((rc = launch(ctx, "k_cont", k_cont, dim3(nb), dim3(256), X, (const int*)ctx->fwd, n, cap,
+                     ctx->box, ctx->brk, ctx->rank, ctx->bsum)))
+      return rc;
+    if ((rc = launch(ctx, "k_scan_blocks", k_scan_blocks, dim3(1), dim3(1024), ctx->bsum, nb, n,
+                     ctx->counts)))
+      return rc;
+    if ((rc = launch(ctx, "k_records", k_records, dim3(nb), dim3(256), X, (const int*)ctx->fwd, n,
+                     cap, ctx->box, (const int*)ctx->brk, (const int*)ctx->rank,
+                     (const int*)ctx->bsum, ctx->near)))
+      return rc;
+    if ((rc = launch(ctx, "k_src_tiles", k_src_tiles, dim3(max_src), dim3(TILE), ctx->near,
+                     (const int*)ctx->counts, ctx->src, ctx->far)))
+      return rc;
+    if ((rc = launch(ctx, "k_tgt_tiles", k_tgt_tiles, dim3(n_tgt), dim3(TILE), X, n, cap, ctx->tgt)))
+      return rc;
+    // enough (target tile, chunk) waves to fill 256 CUs several times over
+    long chunks = (65536 + n_tgt - 1) / n_tgt;
+    chunks = std::max<long>(1, std::min<long>(chunks, 256));
+    size_t pbytes = sizeof(double) * 3 * (size_t)cap * (size_t)chunks;
+    if (pbytes > ctx->partial_bytes) {
+      if ((rc = dalloc(ctx, (void**)&ctx->partial, pbytes))) return rc;
+      ctx->partial_bytes = pbytes;
+    }
+    DirectArgs A;
+    A.P = X;
+    A.cap = cap;
+    A.n = n;
+    A.n_tgt_tiles = n_tgt;
+    A.tgt = ctx->tgt;
+    A.src = ctx->src;
+    A.counts = ctx->counts;
+    A.far = ctx->far;
+    A.near = ctx->near;
+    A.img = ctx->img;
+    A.n_img = n_img;
+    A.mixed = ctx->mixed;
+    A.rnear2 = ctx->rnear * ctx->rnear;
+    A.kappa4pi = ctx->ph.kappa / 4 / 3.14159265358979323846;
+    A.partial = ctx->partial;
+    A.n_chunks = (int)chunks;
+    if ((rc = launch(ctx, "k_direct", k_direct, dim3((unsigned)((n_tgt + 3) / 4), (unsigned)chunks),
+                     dim3(256), A)))
+      return rc;
+    if ((rc = launch(ctx, "k_reduce", k_reduce, grid1(n), dim3(256), (const double*)ctx->partial,
+                     (int)chunks, n, cap, ctx->bs)))
+      return rc;
+    have_bs = 1;
+    double nd = (double)n;
+    ctx->last_interactions = nd * (nd - 2) + (double)(n_img - 1) * nd * nd;
+  } else {
+    ctx->last_interactions = 0;
+  }
+  if ((rc = launch(ctx, "k_epilogue", k_epilogue, grid1(n), dim3(256), (const double*)Tg,
+                   (const double*)Ng, (const double*)ctx->lia, (const double*)ctx->bs, have_bs, n,
+                   cap, ctx->ph, vs, vel, ctx->flag)))
+    return rc;
+  return OV_OK;
+}
+
+// update_tangle(t) + rk4_step(t, dt) on the device-resident state in X0.
+// With caller_k1, K[0] already holds k1 and the t-geometry is not produced.
+int rk4_device(ov_ctx* ctx, double t, double dt, int rk_variant, bool caller_k1) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  int rc;
+  if (!caller_k1) {
+    if ((rc = evaluate(ctx, ctx->X0, true, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0])))
+      return rc;
+  }
+  const double h[3] = {dt / 2, dt / 2, rk_variant == OV_RK_CLASSIC ? dt : dt / 2};
+  const double ts[3] = {t + dt / 2, t + dt / 2, t + dt};
+  for (int st = 0; st < 3; ++st) {
+    if ((rc = launch(ctx, "k_stage", k_stage, grid1(n), dim3(256), (const double*)ctx->X0,
+                     (const double*)ctx->K[st], h[st], n, cap, ctx->Xs)))
+      return rc;
+    (void)ts;
+    if ((rc = evaluate(ctx, ctx->Xs, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[st + 1])))
+      return rc;
+  }
+  return launch(ctx, "k_combine", k_combine, grid1(n), dim3(256), ctx->X0, (const double*)ctx->K[0],
+                (const double*)ctx->K[1], (const double*)ctx->K[2], (const double*)ctx->K[3], dt, n,
+                cap, ctx->flag);
+}
+
+int mark_begin(ov_ctx* ctx) {
+  HIPCHK(hipEventRecord(ctx->t_begin, ctx->stream));
+  return OV_OK;
+}
+
+int finish(ov_ctx* ctx, int* flag_out) {
+  HIPCHK(hipEventRecord(ctx->t_end, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, ctx->t_begin, ctx->t_end) == hipSuccess) ctx->last_device_ms = ms;
+  if (ctx->profile) collect_profile(ctx);
+  int f = 0;
+  HIPCHK(hipMemcpy(&f, ctx->flag, sizeof(int), hipMemcpyDeviceToHost));
+  *flag_out = f;
+  return OV_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+void ov_params_default(ov_params* p) {
+  memset(p, 0, sizeof *p);
+  p->kappa = 9.97e-4;
+  p->vortex_width = 1e-8;
+  p->alpha = 0.1;
+  p->alpha_p = 0.01;
+  p->dl_max = 5e-3;
+  p->LIA_only = 0;
+  p->use_mutual_friction = 1;
+  p->hyperfriction = 1;
+  p->use_BH = 0;
+  p->max_curvature_scale = 0.2;
+  p->hyperalpha = 0.5;
+  p->BH_resolution = 0.1;
+  p->BH_grain = 1e-8;
+  p->BH_quadtree = 0;
+  p->precision = OV_PREC_FP64;
+  p->rk_variant = OV_RK_REFERENCE;
+  p->multipole = OV_MP_REFERENCE;
+  p->near_radius = 0;
+  p->vn.type = p->vs.type = p->vb.type = OV_FLOW_NONE;
+}
+
+ov_ctx* ov_ctx_create(int device) {
+  ov_ctx* ctx = new (std::nothrow) ov_ctx();
+  if (!ctx) return nullptr;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&ctx->flag, 256) != hipSuccess || hipEventCreate(&ctx->t_begin) != hipSuccess ||
+      hipEventCreate(&ctx->t_end) != hipSuccess) {
+    delete ctx;
+    return nullptr;
+  }
+  return ctx;
+}
+
+void ov_ctx_destroy(ov_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  double** v[] = {&ctx->X0, &ctx->Xs, &ctx->K[0], &ctx->K[1], &ctx->K[2], &ctx->K[3], &ctx->T,
+                  &ctx->Nn, &ctx->lia, &ctx->bs,  &ctx->vs,   &ctx->vel_tmp, &ctx->T1, &ctx->N1,
+                  &ctx->vs1, &ctx->dxi, &ctx->dxi1, &ctx->partial};
+  for (double** b : v)
+    if (*b) hipFree(*b);
+  void* o[] = {ctx->fwd,  ctx->rev, ctx->flag, ctx->brk, ctx->rank, ctx->bsum, ctx->counts,
+               ctx->tgt,  ctx->src, ctx->far,  ctx->near, ctx->img};
+  for (void* b : o)
+    if (b) hipFree(b);
+  if (ctx->hst) hipHostFree(ctx->hst);
+  for (auto& p : ctx->pending) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  for (auto e : ctx->ev_pool) hipEventDestroy(e);
+  if (ctx->t_begin) hipEventDestroy(ctx->t_begin);
+  if (ctx->t_end) hipEventDestroy(ctx->t_end);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* ov_last_error(const ov_ctx* ctx) { return ctx ? ctx->err : "null context"; }
+
+void* ov_ctx_stream(ov_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+long ov_ctx_kernel_stats(ov_ctx* ctx, const char* kernel, double* total_ms) {
+  if (!ctx || !kernel) return 0;
+  auto it = ctx->stats.find(kernel);
+  if (it == ctx->stats.end()) {
+    if (total_ms) *total_ms = 0;
+    return 0;
+  }
+  if (total_ms) *total_ms = it->second.ms;
+  return it->second.launches;
+}
+
+void ov_ctx_reset_stats(ov_ctx* ctx) {
+  if (ctx) ctx->stats.clear();
+}
+
+double ov_ctx_last_interactions(const ov_ctx* ctx) { return ctx ? ctx->last_interactions : 0; }
+
+double ov_ctx_last_device_ms(const ov_ctx* ctx) { return ctx ? ctx->last_device_ms : 0; }
+
+int ov_gpu_update_tangents_normals(ov_ctx* ctx, ov_tangle_state* t) {
+  if (!ctx) return OV_E_INVALID;
+  ov_params p;
+  ov_params_default(&p);
+  p.LIA_only = 1;
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = prepare(ctx, t, &p);
+  if (rc) return rc;
+  if (ctx->n == 0) return OV_OK;
+  if ((rc = upload_topology(ctx)) || (rc = upload_vec3(ctx, t->vnodes, ctx->X0))) return rc;
+  if ((rc = mark_begin(ctx))) return rc;
+  if ((rc = launch(ctx, "k_geometry", k_geometry, grid1(ctx->n), dim3(256), (const double*)ctx->X0,
+                   (const int*)ctx->fwd, (const int*)ctx->rev, ctx->n, ctx->cap, ctx->box, ctx->ph, 1,
+                   ctx->T, ctx->Nn, ctx->dxi, ctx->lia)))
+    return rc;
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  if ((rc = download_vec3(ctx, ctx->T, t->tangents)) || (rc = download_vec3(ctx, ctx->Nn, t->normals)) ||
+      (rc = download_scalar(ctx, ctx->dxi, t->dxi)))
+    return rc;
+  return OV_OK;
+}
+
+int ov_gpu_velocities(ov_ctx* ctx, ov_tangle_state* t, double time, const ov_params* p, int flags) {
+  if (!ctx) return OV_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = prepare(ctx, t, p);
+  if (rc) return rc;
+  if (ctx->n == 0) return OV_OK;
+  ctx->profile = (flags & OV_PROFILE) != 0;
+  const bool geom = (flags & OV_UPDATE_GEOMETRY) != 0;
+  if ((rc = upload_topology(ctx)) || (rc = upload_vec3(ctx, t->vnodes, ctx->X0))) return rc;
+  if (!geom) {
+    if (!t->tangents || !t->normals) return fail(ctx, OV_E_INVALID, "tangents/normals required");
+    if ((rc = upload_vec3(ctx, t->tangents, ctx->T)) || (rc = upload_vec3(ctx, t->normals, ctx->Nn)))
+      return rc;
+  }
+  (void)time;
+  if ((rc = mark_begin(ctx))) return rc;
+  if ((rc = evaluate(ctx, ctx->X0, geom, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[0]))) return rc;
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  if ((rc = download_vec3(ctx, ctx->vs, t->vs)) || (rc = download_vec3(ctx, ctx->K[0], t->vels)))
+    return rc;
+  if (geom) {
+    if ((rc = download_vec3(ctx, ctx->T, t->tangents)) || (rc = download_vec3(ctx, ctx->Nn, t->normals)) ||
+        (rc = download_scalar(ctx, ctx->dxi, t->dxi)))
+      return rc;
+  }
+  if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite velocity");
+  return OV_OK;
+}
+
+static int rk4_common(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps,
+                      const ov_params* p, int flags) {
+  if (!ctx) return OV_E_INVALID;
+  if (nsteps < 0) return fail(ctx, OV_E_INVALID, "negative step count");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = prepare(ctx, t, p);
+  if (rc) return rc;
+  if (ctx->n == 0 || nsteps == 0) return OV_OK;
+  ctx->profile = (flags & OV_PROFILE) != 0;
+  const bool caller_k1 = (flags & OV_USE_CALLER_K1) != 0;
+  if ((rc = upload_topology(ctx)) || (rc = upload_vec3(ctx, t->vnodes, ctx->X0))) return rc;
+  if (caller_k1) {
+    if (!t->vels) return fail(ctx, OV_E_INVALID, "vels required as k1");
+    if ((rc = upload_vec3(ctx, t->vels, ctx->K[0]))) return rc;
+  }
+  bool produced_t_state = false;
+  if ((rc = mark_begin(ctx))) return rc;
+  for (int i = 0; i < nsteps; ++i) {
+    bool use_k1 = caller_k1 && i == 0;
+    if ((rc = rk4_device(ctx, t0 + i * dt, dt, p->rk_variant, use_k1))) return rc;
+    produced_t_state = !use_k1;
+    if (flags & OV_ENFORCE_BOUNDARIES) {
+      if ((rc = launch(ctx, "k_wrap", k_wrap, grid1(ctx->n), dim3(256), ctx->X0, ctx->n, ctx->cap,
+                       ctx->box, ctx->wall_periodic_mask)))
+        return rc;
+    }
+  }
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  if ((rc = download_vec3(ctx, ctx->X0, t->vnodes))) return rc;
+  if (produced_t_state) {
+    if ((rc = download_vec3(ctx, ctx->K[0], t->vels)) || (rc = download_vec3(ctx, ctx->vs1, t->vs)) ||
+        (rc = download_vec3(ctx, ctx->T1, t->tangents)) || (rc = download_vec3(ctx, ctx->N1, t->normals)) ||
+        (rc = download_scalar(ctx, ctx->dxi1, t->dxi)))
+      return rc;
+  }
+  if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite %s", (flag & 2) ? "position" : "velocity");
+  return OV_OK;
+}
+
+int ov_gpu_rk4_step(ov_ctx* ctx, ov_tangle_state* t, double time, double dt, const ov_params* p,
+                    int flags) {
+  return rk4_common(ctx, t, time, dt, 1, p, flags & ~OV_ENFORCE_BOUNDARIES);
+}
+
+int ov_gpu_rk4_run(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps,
+                   const ov_params* p, int flags) {
+  return rk4_common(ctx, t, t0, dt, nsteps, p, flags);
+}
+
+}  // extern "C"

@@ -1,0 +1,793 @@
+// ov_kernels.hip — gfx950 kernels of the vortex-filament velocity / RK4 path.
+//
+// Layout (see ov_device.h): SoA FP64 arrays in stream order with stride
+// `cap` between the x, y and z planes; one wavefront per 64-entry tile.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "ov_device.h"
+
+namespace ov {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+// Minimum image of r2 seen from r1 (reference seg_pwrap, vec3_maths.c:105-138):
+// only r2 moves, by one box length per periodic axis beyond half a length.
+__device__ __forceinline__ void pwrap(const double r1[3], const double r2[3], const BoxDev& b,
+                                      double out[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double o = r2[i];
+    if (b.periodic[i]) {
+      double d = r2[i] - r1[i];
+      if (d > b.L[i] / 2)
+        o -= b.L[i];
+      else if (d < -b.L[i] / 2)
+        o += b.L[i];
+    }
+    out[i] = o;
+  }
+}
+
+__device__ __forceinline__ double norm3(const double v[3]) {
+  return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+// |s - minimum image of s'| (reference segment_len of seg_pwrap, tangle.c:536-540)
+__device__ __forceinline__ double chord(const double s[3], const double sp[3], const BoxDev& b) {
+  double w[3], d[3];
+  pwrap(s, sp, b, w);
+  d[0] = s[0] - w[0];
+  d[1] = s[1] - w[1];
+  d[2] = s[2] - w[2];
+  return norm3(d);
+}
+
+__device__ __forceinline__ void load3(const double* __restrict__ P, long cap, int i, double r[3]) {
+  r[0] = P[i];
+  r[1] = P[cap + i];
+  r[2] = P[2 * cap + i];
+}
+__device__ __forceinline__ void store3(double* __restrict__ P, long cap, int i, const double r[3]) {
+  P[i] = r[0];
+  P[cap + i] = r[1];
+  P[2 * cap + i] = r[2];
+}
+
+// FP64 1/sqrt(x) and 1/x: hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  return y;
+}
+__device__ __forceinline__ double rcp64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  return y;
+}
+
+// ---------------------------------------------------------------------------
+// geometry: dxi, s', s'' and the LIA velocity of every point
+// ---------------------------------------------------------------------------
+
+// Five-node Lagrange stencil in cumulative chord length (reference
+// update_tangent_normal free branch, tangle.c:245-273 and 432-479): with
+// nodes x_j (the other three nonzero nodes a, b, c)
+//   w'_j  = -x_a x_b x_c / (x_j (x_j-x_a)(x_j-x_b)(x_j-x_c))
+//   w''_j = 2 (x_a x_b + x_a x_c + x_b x_c) / (x_j (x_j-x_a)(x_j-x_b)(x_j-x_c))
+// applied to the minimum-image differences s_j - s_0.  LIA (tangle.c:530-551):
+//   v_LIA = kappa/4pi ln(2 sqrt(l+ l-) / (e^(1/4) a0)) s' x s''.
+// With compute_geom = 0 the caller's s', s'' (T, Nn) are kept.
+__global__ void __launch_bounds__(256) k_geometry(const double* __restrict__ P,
+                                                  const int* __restrict__ fwd,
+                                                  const int* __restrict__ rev, int n, long cap,
+                                                  BoxDev box, PhysDev ph, int compute_geom,
+                                                  double* __restrict__ T, double* __restrict__ Nn,
+                                                  double* __restrict__ dxi,
+                                                  double* __restrict__ lia) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int f = fwd[s], r = rev[s];
+  double p0[3], pf[3], pr[3];
+  load3(P, cap, s, p0);
+  load3(P, cap, f, pf);
+  load3(P, cap, r, pr);
+  double dk = chord(p0, pf, box);   // dxi[s]
+  double dr = chord(pr, p0, box);   // dxi[rev s]
+  double t[3], nm[3];
+  if (compute_geom) {
+    int ff = fwd[f], rr = rev[r];
+    double pff[3], prr[3];
+    load3(P, cap, ff, pff);
+    load3(P, cap, rr, prr);
+    double df = chord(pf, pff, box);
+    double drr = chord(prr, pr, box);
+    double x[4] = {dk + df, dk, -dr, -(dr + drr)};
+    const double* pj[4] = {pff, pf, pr, prr};
+    t[0] = t[1] = t[2] = 0;
+    nm[0] = nm[1] = nm[2] = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double o[3];
+      int q = 0;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (m != j) o[q++] = x[m];
+      double den = x[j] * (x[j] - o[0]) * (x[j] - o[1]) * (x[j] - o[2]);
+      double w1 = -(o[0] * o[1] * o[2]) / den;
+      double w2 = 2 * (o[0] * o[1] + o[0] * o[2] + o[1] * o[2]) / den;
+      double s0w[3];
+      pwrap(pj[j], p0, box, s0w);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double d = pj[j][i] - s0w[i];
+        t[i] += w1 * d;
+        nm[i] += w2 * d;
+      }
+    }
+    store3(T, cap, s, t);
+    store3(Nn, cap, s, nm);
+    dxi[s] = dk;
+  } else {
+    load3(T, cap, s, t);
+    load3(Nn, cap, s, nm);
+  }
+  const double pi = 3.14159265358979323846;
+  double beta = ph.kappa * log(2 * sqrt(dk * dr) / sqrt(sqrt(2.718281828459045)) / ph.a0) / 4 / pi;
+  double v[3] = {(t[1] * nm[2] - t[2] * nm[1]) * beta, (t[2] * nm[0] - t[0] * nm[2]) * beta,
+                 (t[0] * nm[1] - t[1] * nm[0]) * beta};
+  store3(lia, cap, s, v);
+}
+
+// ---------------------------------------------------------------------------
+// record stream: per-point segment ends, scan, records, tile frames
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block-wide exclusive scan of one int per thread (256 threads).
+__device__ __forceinline__ int block_exscan256(int v, int* lds, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    int t = lds[w];
+    if (w < wid) base += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
+// Segment s -> fwd[s] ends exactly at stream point s+1 ("continues") unless
+// it closes a loop or its end is the periodic image of the forward point
+// (reference seg_pwrap, vec3_maths.c:105-138).  Points whose segment does
+// not continue emit a second record.  Per block: exclusive scan of those
+// flags (rank[s]) and the block total.
+__global__ void __launch_bounds__(256) k_cont(const double* __restrict__ P,
+                                              const int* __restrict__ fwd, int n, long cap,
+                                              BoxDev box, int* __restrict__ brk,
+                                              int* __restrict__ rank, int* __restrict__ bsum) {
+  __shared__ int lds[4];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  int nc = 0;
+  if (s < n) {
+    int f = fwd[s];
+    double a[3], pf[3], b[3];
+    load3(P, cap, s, a);
+    load3(P, cap, f, pf);
+    pwrap(a, pf, box, b);
+    bool cont = (f == s + 1) && b[0] == pf[0] && b[1] == pf[1] && b[2] == pf[2];
+    nc = cont ? 0 : 1;
+    brk[s] = nc;
+  }
+  int tot;
+  int r = block_exscan256(nc, lds, &tot);
+  if (s < n) rank[s] = r;
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the block totals in place (one block of 1024 threads);
+// writes the record count M = n + sum and the source tile count.
+__global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ bsum, int nb, int n,
+                                                      int* __restrict__ counts) {
+  __shared__ int lds[16];
+  __shared__ int carry_s;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int w = 0; w < 16; ++w) {
+      int t = lds[w];
+      if (w < wid) pre += t;
+      tot += t;
+    }
+    int carry = carry_s;
+    if (i < nb) bsum[i] = carry + pre + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int M = n + carry_s;
+    counts[0] = M;
+    counts[1] = (M + TILE - 1) / TILE;
+  }
+}
+
+// Records of point s at offset s + (breaks before s): the point itself,
+// closing segment s-1 when that one continues, then, for a break, the end
+// of segment s (the forward point's minimum image).
+__global__ void __launch_bounds__(256) k_records(const double* __restrict__ P,
+                                                 const int* __restrict__ fwd, int n, long cap,
+                                                 BoxDev box, const int* __restrict__ brk,
+                                                 const int* __restrict__ rank,
+                                                 const int* __restrict__ bsum,
+                                                 NearRec* __restrict__ near) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const int off = s + bsum[blockIdx.x] + rank[s];
+  double a[3];
+  load3(P, cap, s, a);
+  NearRec r0;
+  r0.p[0] = a[0];
+  r0.p[1] = a[1];
+  r0.p[2] = a[2];
+  if (s > 0 && brk[s - 1] == 0) {
+    r0.seg = s - 1;
+    r0.seg_fwd = s;
+  } else {
+    r0.seg = -1;
+    r0.seg_fwd = -1;
+  }
+  near[off] = r0;
+  if (brk[s]) {
+    int f = fwd[s];
+    double pf[3], b[3];
+    load3(P, cap, f, pf);
+    pwrap(a, pf, box, b);
+    NearRec r1;
+    r1.p[0] = b[0];
+    r1.p[1] = b[1];
+    r1.p[2] = b[2];
+    r1.seg = s;
+    r1.seg_fwd = f;
+    near[off + 1] = r1;
+  }
+}
+
+// One 64-thread block per source tile: frame centre = centre of the bounding
+// box of the tile's records and of the record before the tile; FP32 records
+// in that frame with the FP64 segment vector rounded once; padding records
+// beyond M (unlinked, at the centre) complete the last tile.
+__global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
+                                                  const int* __restrict__ counts,
+                                                  SrcTile* __restrict__ tiles,
+                                                  FarRec* __restrict__ far) {
+  const int M = counts[0], nst = counts[1];
+  const int J = blockIdx.x;
+  if (J >= nst) return;
+  const int lane = threadIdx.x;
+  const int e = J * TILE + lane;
+  const bool valid = e < M;
+  NearRec r;
+  if (valid) r = near[e];
+  double prev[3];
+  {
+    int pe = J > 0 ? J * TILE - 1 : 0;
+    NearRec q = near[pe];
+    prev[0] = q.p[0];
+    prev[1] = q.p[1];
+    prev[2] = q.p[2];
+  }
+  double lo[3], hi[3], c[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double v = valid ? r.p[i] : prev[i];
+    double mn = fmin(v, prev[i]), mx = fmax(v, prev[i]);
+    lo[i] = wave_min(mn);
+    hi[i] = wave_max(mx);
+    c[i] = 0.5 * (lo[i] + hi[i]);
+  }
+  // previous record of each lane (lane 0: the record before the tile)
+  double pp[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double v = valid ? r.p[i] : c[i];
+    double up = __shfl_up(v, 1, 64);
+    pp[i] = lane == 0 ? prev[i] : up;
+  }
+  FarRec fr;
+  if (valid) {
+    const bool link = r.seg >= 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      fr.p[i] = (float)(r.p[i] - c[i]);
+      fr.d[i] = link ? (float)(r.p[i] - pp[i]) : 0.f;
+    }
+  } else {
+    NearRec pad;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      fr.p[i] = 0.f;
+      fr.d[i] = 0.f;
+      pad.p[i] = c[i];
+    }
+    pad.seg = -1;
+    pad.seg_fwd = -1;
+    near[e] = pad;
+  }
+  far[e] = fr;
+  if (lane == 0) {
+    SrcTile t;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      t.c[i] = c[i];
+      t.lo[i] = lo[i];
+      t.hi[i] = hi[i];
+      t.prev[i] = prev[i];
+      t.prev_loc[i] = (float)(prev[i] - c[i]);
+    }
+    t.pad = 0;
+    tiles[J] = t;
+  }
+}
+
+// Bounding box of each 64-point target tile.
+__global__ void __launch_bounds__(64) k_tgt_tiles(const double* __restrict__ P, int n, long cap,
+                                                  TgtTile* __restrict__ tiles) {
+  const int T = blockIdx.x;
+  const int s = T * TILE + threadIdx.x;
+  double r[3];
+  if (s < n) {
+    load3(P, cap, s, r);
+  } else {
+    load3(P, cap, T * TILE, r);
+  }
+  TgtTile t;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    t.lo[i] = wave_min(r[i]);
+    t.hi[i] = wave_max(r[i]);
+  }
+  if (threadIdx.x == 0) tiles[T] = t;
+}
+
+// ---------------------------------------------------------------------------
+// direct Biot-Savart sum over all segments and images
+// ---------------------------------------------------------------------------
+
+// Per pair the straight-segment law (reference segment_field1,
+// tangle.c:486-514),
+//   v = kappa/4pi (l + l') / (l l' (l l' + R.R')) R x R',  R = a - r, R' = b - r,
+// is evaluated as (u + u') u u' / (1 + u u' R.R') R x R' with u = 1/l,
+// u' = 1/l'; a record's distance serves the segment it closes and the one it
+// starts.  The reference returns exactly 0 when l or l' < 1e-8 or
+// |R.R'/(l l') - 1| < 1e-8; that rule is kept: pairs whose cheap estimate
+// falls in a narrow band around a threshold are decided by the reference's
+// own FP64 expression (guard_exact).
+
+// Reference decision of the zero-result rules for one pair, evaluated as the
+// reference does (left-to-right sums from 0, no fused multiply-add).
+__device__ __attribute__((noinline)) bool guard_exact(const double a[3], const double b[3],
+                                                      const double r[3]) {
+#pragma clang fp contract(off)
+  double R[3], Rp[3];
+  for (int i = 0; i < 3; ++i) {
+    R[i] = a[i] - r[i];
+    Rp[i] = b[i] - r[i];
+  }
+  double l2 = 0, lp2 = 0, dd = 0;
+  for (int i = 0; i < 3; ++i) {
+    l2 += R[i] * R[i];
+    lp2 += Rp[i] * Rp[i];
+    dd += R[i] * Rp[i];
+  }
+  double l = sqrt(l2), lp = sqrt(lp2);
+  if (l < 1e-8 || lp < 1e-8) return true;
+  double nd = dd / l / lp;
+  return fabs(nd - 1) < 1e-8;
+}
+
+struct DirectArgs {
+  const double* P;  // target positions (SoA, stride cap)
+  long cap;
+  int n;
+  int n_tgt_tiles;
+  const TgtTile* tgt;
+  const SrcTile* src;
+  const int* counts;  // [M, n_src_tiles]
+  const FarRec* far;
+  const NearRec* near;
+  const ImgDev* img;
+  int n_img;
+  int mixed;        // 1: FP32 far field beyond rnear2
+  double rnear2;    // squared box distance of the FP64 near field
+  double kappa4pi;
+  double* partial;  // [n_chunks][3][cap]
+  int n_chunks;
+};
+
+// FP64 pass over one source tile for the image target rs of lane target s.
+__device__ __forceinline__ void near_tile(const NearRec* __restrict__ rec, const SrcTile& st,
+                                          int base, const double rs[3], int s, bool primary,
+                                          double acc[3]) {
+  double ap[3] = {st.prev[0], st.prev[1], st.prev[2]};
+  double Rp[3] = {ap[0] - rs[0], ap[1] - rs[1], ap[2] - rs[2]};
+  double l2p = Rp[0] * Rp[0] + Rp[1] * Rp[1] + Rp[2] * Rp[2];
+  double up = rsq64(l2p);
+#pragma unroll 1
+  for (int e = 0; e < TILE; ++e) {
+    const NearRec q = rec[base + e];
+    double R[3] = {q.p[0] - rs[0], q.p[1] - rs[1], q.p[2] - rs[2]};
+    double l2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+    double u = rsq64(l2);
+    double cx = Rp[1] * R[2] - Rp[2] * R[1];
+    double cy = Rp[2] * R[0] - Rp[0] * R[2];
+    double cz = Rp[0] * R[1] - Rp[1] * R[0];
+    double dab = Rp[0] * R[0] + Rp[1] * R[1] + Rp[2] * R[2];
+    double w = up * u;
+    double tt = fma(w, dab, 1.0);
+    double f = (up + u) * w * rcp64(tt);
+    bool skip = q.seg < 0 || (primary && (q.seg == s || q.seg_fwd == s));
+    bool band = l2p < 1.000001e-16 || l2 < 1.000001e-16 || fabs(tt - 2.0) < 2.00001e-8;
+    if (band && !skip) skip = guard_exact(ap, q.p, rs);
+    if (skip) f = 0.0;
+    acc[0] = fma(f, cx, acc[0]);
+    acc[1] = fma(f, cy, acc[1]);
+    acc[2] = fma(f, cz, acc[2]);
+    Rp[0] = R[0];
+    Rp[1] = R[1];
+    Rp[2] = R[2];
+    l2p = l2;
+    up = u;
+    ap[0] = q.p[0];
+    ap[1] = q.p[1];
+    ap[2] = q.p[2];
+  }
+}
+
+// One FP32 pair: segment from the previous record (R_prev, |R_prev|^2,
+// 1/|R_prev|) to record q; returns the point distance data of q for the next
+// pair.  g = sin^2 - 2e-8 is the zero-rule estimate.
+struct Pair32 {
+  float R[3], l2, u;
+  float c[3], f, g;
+};
+__device__ __forceinline__ Pair32 pair32(const float Rp[3], float l2p, float up, const FarRec& q,
+                                         const float off[3]) {
+  Pair32 o;
+  o.R[0] = q.p[0] + off[0];
+  o.R[1] = q.p[1] + off[1];
+  o.R[2] = q.p[2] + off[2];
+  o.l2 = o.R[0] * o.R[0] + o.R[1] * o.R[1] + o.R[2] * o.R[2];
+  o.u = __builtin_amdgcn_rsqf(o.l2);
+  o.c[0] = Rp[1] * q.d[2] - Rp[2] * q.d[1];
+  o.c[1] = Rp[2] * q.d[0] - Rp[0] * q.d[2];
+  o.c[2] = Rp[0] * q.d[1] - Rp[1] * q.d[0];
+  float dab = l2p + (Rp[0] * q.d[0] + Rp[1] * q.d[1] + Rp[2] * q.d[2]);
+  float w = up * o.u;
+  float tt = fmaf(w, dab, 1.f);
+  float c2 = o.c[0] * o.c[0] + o.c[1] * o.c[1] + o.c[2] * o.c[2];
+  o.g = fmaf(c2, w * w, -2e-8f);
+  o.f = (up + o.u) * w * __builtin_amdgcn_rcpf(tt);
+  return o;
+}
+
+// FP32 pass over one source tile in the tile's frame: R = p_loc + off with
+// off = c_tile - rs rounded once per tile; the cross product is taken with
+// the FP64-rounded segment vector d (R_prev x d = R_prev x R), free of the
+// cancellation of R_prev x R.  Zero rule: 1 - cos < 1e-8 <=> sin^2 < 2e-8
+// here (cos > 0 in the far field), sin^2 = |R_prev x d|^2 u_prev^2 u^2.
+// Records where some lane's estimate lies within a relative 5e-4 of the
+// threshold are noted in a wave-uniform mask and re-decided after the pass
+// with the reference's FP64 expression, correcting the sum where the two
+// decisions differ.
+__device__ __forceinline__ void far_tile(const FarRec* __restrict__ rec,
+                                         const NearRec* __restrict__ nrec, const SrcTile& st,
+                                         int base, const float off[3], const double rs[3],
+                                         float acc[3]) {
+  const float R0[3] = {st.prev_loc[0] + off[0], st.prev_loc[1] + off[1], st.prev_loc[2] + off[2]};
+  float Rp[3] = {R0[0], R0[1], R0[2]};
+  float l2p = Rp[0] * Rp[0] + Rp[1] * Rp[1] + Rp[2] * Rp[2];
+  float up = __builtin_amdgcn_rsqf(l2p);
+  uint64_t bandmask = 0;
+  for (int e0 = 0; e0 < TILE; e0 += 8) {
+    FarRec q[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q[k] = rec[base + e0 + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      Pair32 o = pair32(Rp, l2p, up, q[k], off);
+      float f = o.g < 0.f ? 0.f : o.f;
+      acc[0] = fmaf(f, o.c[0], acc[0]);
+      acc[1] = fmaf(f, o.c[1], acc[1]);
+      acc[2] = fmaf(f, o.c[2], acc[2]);
+      if (__builtin_amdgcn_ballot_w64(fabsf(o.g) < 1e-11f)) bandmask |= 1ull << (e0 + k);
+      Rp[0] = o.R[0];
+      Rp[1] = o.R[1];
+      Rp[2] = o.R[2];
+      l2p = o.l2;
+      up = o.u;
+    }
+  }
+  while (__builtin_expect(bandmask != 0, 0)) {
+    const int e = __builtin_ctzll(bandmask);
+    bandmask &= bandmask - 1;
+    float P[3];
+    if (e == 0) {
+      P[0] = R0[0];
+      P[1] = R0[1];
+      P[2] = R0[2];
+    } else {
+      const FarRec& qp = rec[base + e - 1];
+      P[0] = qp.p[0] + off[0];
+      P[1] = qp.p[1] + off[1];
+      P[2] = qp.p[2] + off[2];
+    }
+    float pl2 = P[0] * P[0] + P[1] * P[1] + P[2] * P[2];
+    float pu = __builtin_amdgcn_rsqf(pl2);
+    Pair32 o = pair32(P, pl2, pu, rec[base + e], off);
+    if (fabsf(o.g) < 1e-11f) {
+      const bool zero32 = o.g < 0.f;
+      const double* a = e > 0 ? nrec[base + e - 1].p : st.prev;
+      const bool zero = guard_exact(a, nrec[base + e].p, rs);
+      if (zero != zero32) {
+        const float f = zero ? -o.f : o.f;
+        acc[0] = fmaf(f, o.c[0], acc[0]);
+        acc[1] = fmaf(f, o.c[1], acc[1]);
+        acc[2] = fmaf(f, o.c[2], acc[2]);
+      }
+    }
+  }
+}
+
+// Grid: blockIdx.x = group of 4 target tiles (one wavefront each),
+// blockIdx.y = chunk of the (source tile, image) work list.  Each wave writes
+// its chunk's sum to partial[chunk]; k_reduce adds the chunks in order.
+__global__ void __launch_bounds__(256) k_direct(DirectArgs A) {
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int T = blockIdx.x * 4 + wid;
+  if (T >= A.n_tgt_tiles) return;
+  const int chunk = blockIdx.y;
+  const long n_work = (long)A.counts[1] * A.n_img;
+  const long w0 = (n_work * chunk) / A.n_chunks;
+  const long w1 = (n_work * (chunk + 1)) / A.n_chunks;
+  const int s = T * TILE + lane;
+  const bool valid = s < A.n;
+  const TgtTile tt = A.tgt[T];
+  double r[3];
+  if (valid) {
+    load3(A.P, A.cap, s, r);
+  } else {
+    load3(A.P, A.cap, T * TILE, r);
+  }
+  double acc[3] = {0, 0, 0};
+  for (long w = w0; w < w1; ++w) {
+    const int J = (int)(w / A.n_img);
+    const int I = (int)(w - (long)J * A.n_img);
+    const ImgDev im = A.img[I];
+    const SrcTile sj = A.src[J];
+    // image target and the target tile's box under the same map
+    double rs[3], tlo[3], thi[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      rs[i] = r[i] - im.shift[i];
+      tlo[i] = tt.lo[i] - im.shift[i];
+      thi[i] = tt.hi[i] - im.shift[i];
+    }
+    if (im.axis >= 0) {
+      const int ax = im.axis;
+      rs[ax] = im.wall2 - r[ax];
+      double l = im.wall2 - tt.hi[ax], h = im.wall2 - tt.lo[ax];
+      tlo[ax] = l;
+      thi[ax] = h;
+    }
+    double d2 = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double g = fmax(0.0, fmax(sj.lo[i] - thi[i], tlo[i] - sj.hi[i]));
+      d2 += g * g;
+    }
+    double v[3] = {0, 0, 0};
+    if (A.mixed && d2 >= A.rnear2) {
+      float off[3], a32[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) off[i] = (float)(sj.c[i] - rs[i]);
+      far_tile(A.far, A.near, sj, J * TILE, off, rs, a32);
+      v[0] = (double)a32[0];
+      v[1] = (double)a32[1];
+      v[2] = (double)a32[2];
+    } else {
+      near_tile(A.near, sj, J * TILE, rs, s, im.primary != 0, v);
+    }
+    if (im.axis >= 0) v[im.axis] = -v[im.axis];
+    acc[0] += v[0];
+    acc[1] += v[1];
+    acc[2] += v[2];
+  }
+  if (valid) {
+    double* out = A.partial + (long)chunk * 3 * A.cap;
+    out[s] = acc[0] * A.kappa4pi;
+    out[A.cap + s] = acc[1] * A.kappa4pi;
+    out[2 * A.cap + s] = acc[2] * A.kappa4pi;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ partial, int n_chunks,
+                                                int n, long cap, double* __restrict__ bs) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double a = 0;
+    for (int c = 0; c < n_chunks; ++c) a += partial[(long)c * 3 * cap + i * cap + s];
+    bs[i * cap + s] = a;
+  }
+}
+// ---------------------------------------------------------------------------
+// per-point epilogue: vs and vels
+// ---------------------------------------------------------------------------
+
+// Reference update_velocity tail (tangle.c:596-683):
+//   vs   = v_LIA + vs_ext + v_BS
+//   vels = vs - [|s''| > c/dl_max] alpha_h s' x v_LIA
+//             + alpha s' x (vn - vs) - alpha' s' x (s' x (vn - vs))
+__global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ T,
+                                                  const double* __restrict__ Nn,
+                                                  const double* __restrict__ lia,
+                                                  const double* __restrict__ bs, int have_bs, int n,
+                                                  long cap, PhysDev ph, double* __restrict__ vs_out,
+                                                  double* __restrict__ vel_out,
+                                                  int* __restrict__ flag) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double t[3], nm[3], l[3], vs[3], v[3];
+  load3(T, cap, s, t);
+  load3(Nn, cap, s, nm);
+  load3(lia, cap, s, l);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) vs[i] = l[i] + ph.vs_ext[i];
+  if (have_bs) {
+    double b[3];
+    load3(bs, cap, s, b);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) vs[i] += b[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = vs[i];
+  if (ph.hyperfriction) {
+    double spp = norm3(nm);
+    if (spp > ph.max_curv / ph.dl_max) {
+      double h[3] = {t[1] * l[2] - t[2] * l[1], t[2] * l[0] - t[0] * l[2], t[0] * l[1] - t[1] * l[0]};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) v[i] += h[i] * -ph.hyperalpha;
+    }
+  }
+  if (ph.use_mf) {
+    double dv[3] = {ph.vn_ext[0] - vs[0], ph.vn_ext[1] - vs[1], ph.vn_ext[2] - vs[2]};
+    double a1[3] = {t[1] * dv[2] - t[2] * dv[1], t[2] * dv[0] - t[0] * dv[2],
+                    t[0] * dv[1] - t[1] * dv[0]};
+    double a2[3] = {t[1] * a1[2] - t[2] * a1[1], t[2] * a1[0] - t[0] * a1[2],
+                    t[0] * a1[1] - t[1] * a1[0]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      v[i] += a1[i] * ph.alpha;
+      v[i] += a2[i] * -ph.alpha_p;
+    }
+  }
+  store3(vs_out, cap, s, vs);
+  store3(vel_out, cap, s, v);
+  bool bad = !(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]) && isfinite(vs[0]) &&
+               isfinite(vs[1]) && isfinite(vs[2]));
+  if (bad) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// RK4 stages (reference vortex_dynamics.c:36-52, 58-117)
+// ---------------------------------------------------------------------------
+
+// Xs = X0 + h K (euler_step2: move = v dt, then x + move)
+__global__ void __launch_bounds__(256) k_stage(const double* __restrict__ X0,
+                                               const double* __restrict__ K, double h, int n,
+                                               long cap, double* __restrict__ Xs) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double mv = K[i * cap + s] * h;
+    Xs[i * cap + s] = X0[i * cap + s] + mv;
+  }
+}
+
+// X0 += dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4, accumulated in that order.
+__global__ void __launch_bounds__(256) k_combine(double* __restrict__ X0,
+                                                 const double* __restrict__ K1,
+                                                 const double* __restrict__ K2,
+                                                 const double* __restrict__ K3,
+                                                 const double* __restrict__ K4, double dt, int n,
+                                                 long cap, int* __restrict__ flag) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  bool bad = false;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    long o = i * cap + s;
+    double mv = K1[o] * (dt / 6);
+    mv = mv + K2[o] * (dt / 3);
+    mv = mv + K3[o] * (dt / 3);
+    mv = mv + K4[o] * (dt / 6);
+    double x = X0[o] + mv;
+    X0[o] = x;
+    bad |= !isfinite(x);
+  }
+  if (bad) atomicOr(flag, 2);
+}
+
+// Periodic wrap (reference enforce_boundaries, tangle.c:766-819): while the
+// point is beyond a periodic face (the last offending face in x, y, z order),
+// move it by one box length.  Non-finite points are left alone.
+__global__ void __launch_bounds__(256) k_wrap(double* __restrict__ X0, int n, long cap, BoxDev box,
+                                              int wall_periodic_mask) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double x[3];
+  load3(X0, cap, s, x);
+  if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) return;
+  for (int it = 0; it < 64; ++it) {
+    int face = -1;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      if (x[ax] < box.lo[ax]) face = 2 * ax;
+      if (x[ax] > box.hi[ax]) face = 2 * ax + 1;
+    }
+    if (face < 0 || !((wall_periodic_mask >> face) & 1)) break;
+    int ax = face / 2;
+    if (face & 1)
+      x[ax] -= box.L[ax];
+    else
+      x[ax] += box.L[ax];
+  }
+  store3(X0, cap, s, x);
+}
+
+}  // namespace ov

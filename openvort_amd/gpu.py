@@ -1,0 +1,152 @@
+"""ctypes binding of libopenvort_gpu.so (include/openvort_gpu.h).
+
+The product path: every call here runs the HIP kernels.  Loading fails
+loudly when the library is missing — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .tangle import Params, Tangle, TangleState
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopenvort_gpu.so")
+
+OV_OK, OV_E_INVALID, OV_E_UNSUPPORTED, OV_E_NONFINITE, OV_E_DEVICE, OV_E_NOMEM = 0, -1, -2, -3, -4, -5
+OV_UPDATE_GEOMETRY, OV_USE_CALLER_K1, OV_ENFORCE_BOUNDARIES, OV_PROFILE = 1, 2, 4, 8
+
+# every symbol the header declares
+SYMBOLS = (
+    "ov_params_default",
+    "ov_ctx_create",
+    "ov_ctx_destroy",
+    "ov_last_error",
+    "ov_ctx_stream",
+    "ov_gpu_update_tangents_normals",
+    "ov_gpu_velocities",
+    "ov_gpu_rk4_step",
+    "ov_gpu_rk4_run",
+    "ov_ctx_kernel_stats",
+    "ov_ctx_reset_stats",
+    "ov_ctx_last_interactions",
+    "ov_ctx_last_device_ms",
+)
+
+_lib = None
+
+
+class OVError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"openvort_gpu error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load and type the library (no device calls)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with python -m openvort_amd.build")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    lib.ov_params_default.argtypes = [P(Params)]
+    lib.ov_params_default.restype = None
+    lib.ov_ctx_create.argtypes = [C.c_int]
+    lib.ov_ctx_create.restype = C.c_void_p
+    lib.ov_ctx_destroy.argtypes = [C.c_void_p]
+    lib.ov_ctx_destroy.restype = None
+    lib.ov_last_error.argtypes = [C.c_void_p]
+    lib.ov_last_error.restype = C.c_char_p
+    lib.ov_ctx_stream.argtypes = [C.c_void_p]
+    lib.ov_ctx_stream.restype = C.c_void_p
+    lib.ov_gpu_update_tangents_normals.argtypes = [C.c_void_p, P(TangleState)]
+    lib.ov_gpu_update_tangents_normals.restype = C.c_int
+    lib.ov_gpu_velocities.argtypes = [C.c_void_p, P(TangleState), C.c_double, P(Params), C.c_int]
+    lib.ov_gpu_velocities.restype = C.c_int
+    lib.ov_gpu_rk4_step.argtypes = [C.c_void_p, P(TangleState), C.c_double, C.c_double, P(Params), C.c_int]
+    lib.ov_gpu_rk4_step.restype = C.c_int
+    lib.ov_gpu_rk4_run.argtypes = [C.c_void_p, P(TangleState), C.c_double, C.c_double, C.c_int,
+                                   P(Params), C.c_int]
+    lib.ov_gpu_rk4_run.restype = C.c_int
+    lib.ov_ctx_kernel_stats.argtypes = [C.c_void_p, C.c_char_p, P(C.c_double)]
+    lib.ov_ctx_kernel_stats.restype = C.c_long
+    lib.ov_ctx_reset_stats.argtypes = [C.c_void_p]
+    lib.ov_ctx_reset_stats.restype = None
+    lib.ov_ctx_last_interactions.argtypes = [C.c_void_p]
+    lib.ov_ctx_last_interactions.restype = C.c_double
+    lib.ov_ctx_last_device_ms.argtypes = [C.c_void_p]
+    lib.ov_ctx_last_device_ms.restype = C.c_double
+    _lib = lib
+    return lib
+
+
+class Context:
+    """One device context (stream + buffers) of libopenvort_gpu."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self.ptr = self.lib.ov_ctx_create(device)
+        if not self.ptr:
+            raise OVError(OV_E_DEVICE, f"ov_ctx_create({device}) failed (no HIP device?)")
+
+    def close(self):
+        if self.ptr:
+            self.lib.ov_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, allow_nonfinite=False) -> int:
+        if rc == OV_OK or (allow_nonfinite and rc == OV_E_NONFINITE):
+            return rc
+        raise OVError(rc, self.lib.ov_last_error(self.ptr).decode(errors="replace"))
+
+    # -- entry points ------------------------------------------------------
+    def update_tangents_normals(self, t: Tangle) -> int:
+        ts = t.cstruct()
+        return self._check(self.lib.ov_gpu_update_tangents_normals(self.ptr, C.byref(ts)))
+
+    def velocities(self, t: Tangle, time: float, params: Params, geometry=True, profile=False,
+                   allow_nonfinite=False) -> int:
+        ts = t.cstruct()
+        flags = (OV_UPDATE_GEOMETRY if geometry else 0) | (OV_PROFILE if profile else 0)
+        rc = self.lib.ov_gpu_velocities(self.ptr, C.byref(ts), time, C.byref(params), flags)
+        return self._check(rc, allow_nonfinite)
+
+    def rk4_step(self, t: Tangle, time: float, dt: float, params: Params, caller_k1=False,
+                 profile=False, allow_nonfinite=False) -> int:
+        ts = t.cstruct()
+        flags = (OV_USE_CALLER_K1 if caller_k1 else 0) | (OV_PROFILE if profile else 0)
+        rc = self.lib.ov_gpu_rk4_step(self.ptr, C.byref(ts), time, dt, C.byref(params), flags)
+        return self._check(rc, allow_nonfinite)
+
+    def rk4_run(self, t: Tangle, t0: float, dt: float, nsteps: int, params: Params,
+                enforce_boundaries=False, profile=False, allow_nonfinite=False) -> int:
+        ts = t.cstruct()
+        flags = (OV_ENFORCE_BOUNDARIES if enforce_boundaries else 0) | (OV_PROFILE if profile else 0)
+        rc = self.lib.ov_gpu_rk4_run(self.ptr, C.byref(ts), t0, dt, nsteps, C.byref(params), flags)
+        return self._check(rc, allow_nonfinite)
+
+    # -- profiling -----------------------------------------------------------
+    def kernel_stats(self, name: str):
+        ms = C.c_double(0)
+        n = self.lib.ov_ctx_kernel_stats(self.ptr, name.encode(), C.byref(ms))
+        return int(n), float(ms.value)
+
+    def reset_stats(self):
+        self.lib.ov_ctx_reset_stats(self.ptr)
+
+    def last_interactions(self) -> float:
+        return float(self.lib.ov_ctx_last_interactions(self.ptr))
+
+    def last_device_ms(self) -> float:
+        return float(self.lib.ov_ctx_last_device_ms(self.ptr))
+
+    def stream(self) -> int:
+        return int(self.lib.ov_ctx_stream(self.ptr) or 0)

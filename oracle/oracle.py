@@ -1,0 +1,182 @@
+"""ctypes access to the test-only checkers.
+
+TEST INFRASTRUCTURE ONLY — imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of bench.py, never by the product package.
+
+* ``Oracle``: the FP64 C restatement (liboracle.so, ov_oracle.c).
+* ``Reference``: the reference's own hot-path C compiled from its sources
+  (oracle/_ref/libovref.so, built by oracle/Makefile where the reference
+  tree exists); ``Reference.available()`` is False elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(_HERE))
+
+from openvort_amd.tangle import Params, Tangle, TangleState  # noqa: E402
+
+ORACLE_LIB = os.path.join(_HERE, "liboracle.so")
+REF_LIB = os.path.join(_HERE, "_ref", "libovref.so")
+
+P = C.POINTER
+
+
+class Oracle:
+    _lib = None
+
+    def __init__(self, threads: int = 0):
+        if Oracle._lib is None:
+            if not os.path.exists(ORACLE_LIB):
+                raise RuntimeError(f"{ORACLE_LIB} missing: run make -C oracle")
+            lib = C.CDLL(ORACLE_LIB)
+            lib.ovo_update_tangents_normals.argtypes = [P(TangleState)]
+            lib.ovo_update_velocities.argtypes = [P(TangleState), C.c_double, P(Params)]
+            lib.ovo_velocities_sample.argtypes = [P(TangleState), C.c_double, P(Params), P(C.c_int),
+                                                  C.c_int, P(C.c_double), P(C.c_double)]
+            lib.ovo_rk4_step.argtypes = [P(TangleState), C.c_double, C.c_double, P(Params)]
+            lib.ovo_enforce_boundaries.argtypes = [P(TangleState)]
+            lib.ovo_enforce_boundaries.restype = None
+            lib.ovo_segment_field.argtypes = [P(C.c_double), P(C.c_double), P(C.c_double), C.c_double,
+                                              P(C.c_double)]
+            lib.ovo_segment_field.restype = None
+            lib.ovo_octree_stats.argtypes = [P(C.c_long), P(C.c_long), P(C.c_long)]
+            lib.ovo_octree_stats.restype = None
+            lib.ovo_set_threads.argtypes = [C.c_int]
+            lib.ovo_set_threads.restype = None
+            Oracle._lib = lib
+        self.lib = Oracle._lib
+        self.lib.ovo_set_threads(threads)
+
+    def _rc(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"oracle returned {rc}")
+
+    def update_tangents_normals(self, t: Tangle):
+        ts = t.cstruct()
+        self._rc(self.lib.ovo_update_tangents_normals(C.byref(ts)))
+
+    def update_velocities(self, t: Tangle, time: float, p: Params):
+        ts = t.cstruct()
+        self._rc(self.lib.ovo_update_velocities(C.byref(ts), time, C.byref(p)))
+
+    def update_tangle(self, t: Tangle, time: float, p: Params):
+        self.update_tangents_normals(t)
+        self.update_velocities(t, time, p)
+
+    def velocities_sample(self, t: Tangle, time: float, p: Params, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        vs = np.zeros((len(idx), 3))
+        vel = np.zeros((len(idx), 3))
+        ts = t.cstruct()
+        self._rc(self.lib.ovo_velocities_sample(C.byref(ts), time, C.byref(p),
+                                                idx.ctypes.data_as(P(C.c_int)), len(idx),
+                                                vs.ctypes.data_as(P(C.c_double)),
+                                                vel.ctypes.data_as(P(C.c_double))))
+        return vs, vel
+
+    def rk4_step(self, t: Tangle, time: float, dt: float, p: Params):
+        """rk4_step alone: k1 = t.vels as given."""
+        ts = t.cstruct()
+        self._rc(self.lib.ovo_rk4_step(C.byref(ts), time, dt, C.byref(p)))
+
+    def step(self, t: Tangle, time: float, dt: float, p: Params):
+        """update_tangle(t) then rk4_step(t, dt) (reference main.c:112-113)."""
+        self.update_tangle(t, time, p)
+        self.rk4_step(t, time, dt, p)
+
+    def enforce_boundaries(self, t: Tangle):
+        ts = t.cstruct()
+        self.lib.ovo_enforce_boundaries(C.byref(ts))
+
+    def segment_field(self, a, b, r, kappa):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros(3)
+        self.lib.ovo_segment_field(a.ctypes.data_as(P(C.c_double)), b.ctypes.data_as(P(C.c_double)),
+                                   r.ctypes.data_as(P(C.c_double)), kappa,
+                                   out.ctypes.data_as(P(C.c_double)))
+        return out
+
+    def octree_stats(self):
+        a, b, c = C.c_long(), C.c_long(), C.c_long()
+        self.lib.ovo_octree_stats(C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+
+class Reference:
+    """The reference's own C (update_tangents_normals, update_velocities,
+    update_velocity, rk4_step, enforce_boundaries) through ref_harness.c."""
+
+    _lib = None
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_LIB)
+
+    def __init__(self, threads: int = 1):
+        if Reference._lib is None:
+            lib = C.CDLL(REF_LIB)
+            lib.ovref_set_params.argtypes = [P(Params), C.c_int]
+            lib.ovref_update_tangents_normals.argtypes = [P(TangleState)]
+            lib.ovref_update_tangents_normals.restype = None
+            lib.ovref_update_velocities.argtypes = [P(TangleState), C.c_double]
+            lib.ovref_update_velocities.restype = None
+            lib.ovref_velocities_sample.argtypes = [P(TangleState), C.c_double, P(C.c_int), C.c_int,
+                                                    P(C.c_double), P(C.c_double)]
+            lib.ovref_velocities_sample.restype = None
+            lib.ovref_rk4_step.argtypes = [P(TangleState), C.c_double, C.c_double]
+            lib.ovref_rk4_step.restype = None
+            lib.ovref_enforce_boundaries.argtypes = [P(TangleState)]
+            lib.ovref_enforce_boundaries.restype = None
+            Reference._lib = lib
+        self.lib = Reference._lib
+        self.threads = threads
+
+    def set_params(self, p: Params):
+        rc = self.lib.ovref_set_params(C.byref(p), self.threads)
+        if rc:
+            raise RuntimeError("reference: unsupported flow")
+
+    def update_tangents_normals(self, t: Tangle):
+        ts = t.cstruct()
+        self.lib.ovref_update_tangents_normals(C.byref(ts))
+
+    def update_velocities(self, t: Tangle, time: float, p: Params):
+        self.set_params(p)
+        ts = t.cstruct()
+        self.lib.ovref_update_velocities(C.byref(ts), time)
+
+    def update_tangle(self, t: Tangle, time: float, p: Params):
+        self.update_tangents_normals(t)
+        self.update_velocities(t, time, p)
+
+    def velocities_sample(self, t: Tangle, time: float, p: Params, idx):
+        self.set_params(p)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        vs = np.zeros((len(idx), 3))
+        vel = np.zeros((len(idx), 3))
+        ts = t.cstruct()
+        self.lib.ovref_velocities_sample(C.byref(ts), time, idx.ctypes.data_as(P(C.c_int)), len(idx),
+                                         vs.ctypes.data_as(P(C.c_double)),
+                                         vel.ctypes.data_as(P(C.c_double)))
+        return vs, vel
+
+    def rk4_step(self, t: Tangle, time: float, dt: float, p: Params):
+        self.set_params(p)
+        ts = t.cstruct()
+        self.lib.ovref_rk4_step(C.byref(ts), time, dt)
+
+    def step(self, t: Tangle, time: float, dt: float, p: Params):
+        self.update_tangle(t, time, p)
+        self.rk4_step(t, time, dt, p)
+
+    def enforce_boundaries(self, t: Tangle):
+        ts = t.cstruct()
+        self.lib.ovref_enforce_boundaries(C.byref(ts))

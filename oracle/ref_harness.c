@@ -1,0 +1,118 @@
+/*
+ * ref_harness.c — drives the compiled reference (emil-varga/openvort) from
+ * ctypes.  TEST INFRASTRUCTURE ONLY: linked with the reference's own
+ * src/*.c (oracle/Makefile) into oracle/_ref/libovref.so, which pins the
+ * restatement in ov_oracle.c and can serve as the timed CPU baseline.
+ *
+ * The reference's globals (src/vortex_constants.c:24-95) and flow configs
+ * (src/external_velocity.c:76-78) are set from an ov_params; the tangle is
+ * passed straight through, ov_tangle_state being layout-identical to
+ * struct tangle_state (include/tangle.h:82-106; checked below).
+ */
+#include <stddef.h>
+#include <string.h>
+
+#include "external_velocity.h"
+#include "octree.h"
+#include "tangle.h"
+#include "vortex_constants.h"
+#include "vortex_dynamics.h"
+
+#include "../include/openvort_gpu.h"
+
+_Static_assert(sizeof(ov_tangle_state) == sizeof(struct tangle_state), "layout");
+_Static_assert(offsetof(ov_tangle_state, connections) == offsetof(struct tangle_state, connections), "layout");
+_Static_assert(offsetof(ov_tangle_state, box) == offsetof(struct tangle_state, box), "layout");
+_Static_assert(offsetof(ov_tangle_state, bimg) == offsetof(struct tangle_state, bimg), "layout");
+_Static_assert(offsetof(ov_tangle_state, N) == offsetof(struct tangle_state, N), "layout");
+_Static_assert(sizeof(ov_node_status) == sizeof(node_status), "layout");
+_Static_assert(sizeof(ov_image_tangle) == sizeof(struct image_tangle), "layout");
+
+extern int global_num_threads;
+
+static int find_conf(const char *name) {
+  for (int k = 0; v_confs[k].fun; ++k)
+    if (strcmp(v_confs[k].name, name) == 0) return k;
+  return -1;
+}
+
+static int set_flow(struct v_conf_t *dst, const ov_flow *f) {
+  if (f->type == OV_FLOW_NONE) {
+    *dst = v_confs[find_conf("no flow")];
+    return 0;
+  }
+  if (f->type == OV_FLOW_SIMPLE) {
+    *dst = v_confs[find_conf("simple")];
+    dst->v_params[0].value.vector = vec3(f->v[0], f->v[1], f->v[2]);
+    return 0;
+  }
+  return -1;
+}
+
+/* Apply an ov_params to the reference globals. */
+int ovref_set_params(const ov_params *p, int threads) {
+  KAPPA = p->kappa;
+  VORTEX_WIDTH = p->vortex_width;
+  global_alpha = p->alpha;
+  global_alpha_p = p->alpha_p;
+  global_dl_max = p->dl_max;
+  global_LIA_only = p->LIA_only;
+  global_use_mutual_friction = p->use_mutual_friction;
+  global_hyperfriction = p->hyperfriction;
+  global_max_curvature_scale = p->max_curvature_scale;
+  global_hyperalpha = p->hyperalpha;
+  global_use_BH = p->use_BH;
+  global_BH_resolution = p->BH_resolution;
+  global_BH_grain = p->BH_grain;
+  global_BH_quadtree = p->BH_quadtree;
+  global_num_threads = threads > 0 ? threads : 1;
+  if (set_flow(&vn_conf, &p->vn) || set_flow(&vs_conf, &p->vs) || set_flow(&vb_conf, &p->vb))
+    return -2;
+  return 0;
+}
+
+/* update_tangents_normals (tangle.c:723-727). */
+void ovref_update_tangents_normals(ov_tangle_state *t) {
+  update_tangents_normals((struct tangle_state *)t);
+}
+
+/* update_velocities (tangle.c:686-702), octree built inside when use_BH. */
+void ovref_update_velocities(ov_tangle_state *t, double time) {
+  update_velocities((struct tangle_state *)t, time, NULL);
+}
+
+/* update_velocity for the listed points only, OpenMP-parallel like
+ * update_velocities; results copied to [n][3] arrays. */
+void ovref_velocities_sample(ov_tangle_state *t, double time, const int *idx, int n,
+                             double *vs_out, double *vels_out) {
+  struct tangle_state *ts = (struct tangle_state *)t;
+  struct octree *tree = global_use_BH ? octree_build(ts, global_BH_quadtree) : NULL;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(global_num_threads)
+  for (int q = 0; q < n; ++q) {
+    int k = idx[q];
+    struct vec3d vs, vel;
+    if (ts->status[k].status == EMPTY) {
+      vs = vec3(0, 0, 0);
+      vel = vs;
+    } else {
+      /* update_velocity writes into the tangle arrays at k; the caller gives
+       * distinct k, so the OpenMP loop is race-free as in tangle.c:693-699 */
+      update_velocity(ts, k, time, tree);
+      vs = ts->vs[k];
+      vel = ts->vels[k];
+    }
+    memcpy(vs_out + 3 * q, vs.p, sizeof vs.p);
+    memcpy(vels_out + 3 * q, vel.p, sizeof vel.p);
+  }
+  if (tree) octree_destroy(tree);
+}
+
+/* rk4_step (vortex_dynamics.c:119-121): k1 = t->vels. */
+void ovref_rk4_step(ov_tangle_state *t, double time, double dt) {
+  rk4_step((struct tangle_state *)t, time, dt);
+}
+
+/* enforce_boundaries (tangle.c:796-819). */
+void ovref_enforce_boundaries(ov_tangle_state *t) {
+  enforce_boundaries((struct tangle_state *)t);
+}
