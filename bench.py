@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--dt", type=float, default=1e-6)
-    ap.add_argument("--near-radius", type=float, default=0.005)
+    ap.add_argument("--near-radius", type=float, default=0.02)
     ap.add_argument("--fp64", type=int, default=1, help="FP64 evaluations for the comparison (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=64, help="targets of the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)

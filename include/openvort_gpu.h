@@ -102,7 +102,7 @@ typedef struct {
   int precision;              /* OV_PREC_*   (no reference counterpart) */
   int rk_variant;             /* OV_RK_*     (no reference counterpart) */
   int multipole;              /* OV_MP_*     (no reference counterpart) */
-  double near_radius;         /* mixed: FP64 below this box distance, <=0: default */
+  double near_radius;         /* mixed: FP64 below this box distance (cm), <=0: 0.02 */
   ov_flow vn, vs, vb;         /* vn_conf, vs_conf, vb_conf   external_velocity.c:76-78 */
 } ov_params;
 

@@ -50,7 +50,7 @@ struct ov_ctx {
   std::vector<ImgDev> imgs;
   PhysDev ph{};
   int mixed = 0;
-  double rnear = 0.005;
+  double rnear = 0.02;
   int wall_periodic_mask = 0;
   bool lia_only = false;
 
@@ -319,7 +319,7 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   ph.hyperfriction = p->hyperfriction;
   ctx->lia_only = p->LIA_only != 0;
   ctx->mixed = p->precision == OV_PREC_MIXED;
-  ctx->rnear = p->near_radius > 0 ? p->near_radius : 0.005;
+  ctx->rnear = p->near_radius > 0 ? p->near_radius : 0.02;
   ctx->profile = false;
 
   int rc = ensure_capacity(ctx, std::max(n, 1));

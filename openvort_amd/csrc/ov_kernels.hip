@@ -407,8 +407,8 @@ __global__ void __launch_bounds__(64) k_tgt_tiles(const double* __restrict__ P, 
 
 // Reference decision of the zero-result rules for one pair, evaluated as the
 // reference does (left-to-right sums from 0, no fused multiply-add).
-__device__ __attribute__((noinline)) bool guard_exact(const double a[3], const double b[3],
-                                                      const double r[3]) {
+__device__ __forceinline__ bool guard_exact(const double a[3], const double b[3],
+                                            const double r[3]) {
 #pragma clang fp contract(off)
   double R[3], Rp[3];
   for (int i = 0; i < 3; ++i) {
@@ -446,42 +446,102 @@ struct DirectArgs {
   int n_chunks;
 };
 
+// One FP64 pair: segment from the previous record (R_prev, |R_prev|^2,
+// 1/|R_prev|) to the record at p, for the image target rs; returns the
+// distance data of p for the next pair.  nd1 = |cos - 1| estimates the
+// reference's |R.R'/(l l') - 1|.
+struct Pair64 {
+  double R[3], l2, u;
+  double c[3], f, nd1;
+};
+__device__ __forceinline__ Pair64 pair64(const double Rp[3], double l2p, double up,
+                                         const double p[3], const double rs[3]) {
+  Pair64 o;
+  o.R[0] = p[0] - rs[0];
+  o.R[1] = p[1] - rs[1];
+  o.R[2] = p[2] - rs[2];
+  o.l2 = o.R[0] * o.R[0] + o.R[1] * o.R[1] + o.R[2] * o.R[2];
+  o.u = rsq64(o.l2);
+  o.c[0] = Rp[1] * o.R[2] - Rp[2] * o.R[1];
+  o.c[1] = Rp[2] * o.R[0] - Rp[0] * o.R[2];
+  o.c[2] = Rp[0] * o.R[1] - Rp[1] * o.R[0];
+  double dab = Rp[0] * o.R[0] + Rp[1] * o.R[1] + Rp[2] * o.R[2];
+  double w = up * o.u;
+  double tt = fma(w, dab, 1.0);
+  o.nd1 = fabs(tt - 2.0);
+  o.f = (up + o.u) * w * rcp64(tt);
+  return o;
+}
+
 // FP64 pass over one source tile for the image target rs of lane target s.
+// Zero rules (l, l' < 1e-8; |cos - 1| < 1e-8) are decided on the estimate,
+// which carries ~1e-15 error: records where some lane's estimate lies within
+// 1e-13 of the cosine threshold or 1e-6 relative of l^2 = 1e-16 are noted
+// in a wave-uniform mask (and, per lane, the band flag and the decision
+// taken) and re-decided after the pass with the reference's own expression,
+// correcting the sum where the decisions differ.
 __device__ __forceinline__ void near_tile(const NearRec* __restrict__ rec, const SrcTile& st,
                                           int base, const double rs[3], int s, bool primary,
                                           double acc[3]) {
-  double ap[3] = {st.prev[0], st.prev[1], st.prev[2]};
-  double Rp[3] = {ap[0] - rs[0], ap[1] - rs[1], ap[2] - rs[2]};
+  const double R0[3] = {st.prev[0] - rs[0], st.prev[1] - rs[1], st.prev[2] - rs[2]};
+  double Rp[3] = {R0[0], R0[1], R0[2]};
   double l2p = Rp[0] * Rp[0] + Rp[1] * Rp[1] + Rp[2] * Rp[2];
   double up = rsq64(l2p);
-#pragma unroll 1
+  uint64_t bandmask = 0, lane_band = 0, lane_zero = 0;
+#pragma unroll 2
   for (int e = 0; e < TILE; ++e) {
     const NearRec q = rec[base + e];
-    double R[3] = {q.p[0] - rs[0], q.p[1] - rs[1], q.p[2] - rs[2]};
-    double l2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
-    double u = rsq64(l2);
-    double cx = Rp[1] * R[2] - Rp[2] * R[1];
-    double cy = Rp[2] * R[0] - Rp[0] * R[2];
-    double cz = Rp[0] * R[1] - Rp[1] * R[0];
-    double dab = Rp[0] * R[0] + Rp[1] * R[1] + Rp[2] * R[2];
-    double w = up * u;
-    double tt = fma(w, dab, 1.0);
-    double f = (up + u) * w * rcp64(tt);
-    bool skip = q.seg < 0 || (primary && (q.seg == s || q.seg_fwd == s));
-    bool band = l2p < 1.000001e-16 || l2 < 1.000001e-16 || fabs(tt - 2.0) < 2.00001e-8;
-    if (band && !skip) skip = guard_exact(ap, q.p, rs);
-    if (skip) f = 0.0;
-    acc[0] = fma(f, cx, acc[0]);
-    acc[1] = fma(f, cy, acc[1]);
-    acc[2] = fma(f, cz, acc[2]);
-    Rp[0] = R[0];
-    Rp[1] = R[1];
-    Rp[2] = R[2];
-    l2p = l2;
-    up = u;
-    ap[0] = q.p[0];
-    ap[1] = q.p[1];
-    ap[2] = q.p[2];
+    Pair64 o = pair64(Rp, l2p, up, q.p, rs);
+    const bool skip = q.seg < 0 || (primary && (q.seg == s || q.seg_fwd == s));
+    const bool zero = l2p < 1e-16 || o.l2 < 1e-16 || o.nd1 < 1e-8;
+    const bool band = fabs(l2p - 1e-16) < 1e-22 || fabs(o.l2 - 1e-16) < 1e-22 ||
+                      fabs(o.nd1 - 1e-8) < 1e-13;
+    const double f = (skip || zero) ? 0.0 : o.f;
+    acc[0] = fma(f, o.c[0], acc[0]);
+    acc[1] = fma(f, o.c[1], acc[1]);
+    acc[2] = fma(f, o.c[2], acc[2]);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(band && !skip) != 0, 0)) {
+      bandmask |= 1ull << e;
+      if (band && !skip) lane_band |= 1ull << e;
+      if (band && !skip && zero) lane_zero |= 1ull << e;
+    }
+    Rp[0] = o.R[0];
+    Rp[1] = o.R[1];
+    Rp[2] = o.R[2];
+    l2p = o.l2;
+    up = o.u;
+  }
+  while (__builtin_expect(bandmask != 0, 0)) {
+    const int e = __builtin_ctzll(bandmask);
+    bandmask &= bandmask - 1;
+    double a[3], P[3];
+    const NearRec q = rec[base + e];
+    if (e == 0) {
+      a[0] = st.prev[0];
+      a[1] = st.prev[1];
+      a[2] = st.prev[2];
+    } else {
+      const NearRec qp = rec[base + e - 1];
+      a[0] = qp.p[0];
+      a[1] = qp.p[1];
+      a[2] = qp.p[2];
+    }
+    P[0] = a[0] - rs[0];
+    P[1] = a[1] - rs[1];
+    P[2] = a[2] - rs[2];
+    const double pl2 = P[0] * P[0] + P[1] * P[1] + P[2] * P[2];
+    const double pu = rsq64(pl2);
+    Pair64 o = pair64(P, pl2, pu, q.p, rs);
+    if ((lane_band >> e) & 1) {
+      const bool zero_est = (lane_zero >> e) & 1;
+      const bool zero = guard_exact(a, q.p, rs);
+      if (zero != zero_est) {
+        const double f = zero ? -o.f : o.f;
+        acc[0] = fma(f, o.c[0], acc[0]);
+        acc[1] = fma(f, o.c[1], acc[1]);
+        acc[2] = fma(f, o.c[2], acc[2]);
+      }
+    }
   }
 }
 
@@ -513,41 +573,51 @@ __device__ __forceinline__ Pair32 pair32(const float Rp[3], float l2p, float up,
 }
 
 // FP32 pass over one source tile in the tile's frame: R = p_loc + off with
-// off = c_tile - rs rounded once per tile; the cross product is taken with
+// off = c_tile - rs rounded once per tile; FP32 partial sums of 8 records are
+// added into the lane's FP64 sum; the cross product is taken with
 // the FP64-rounded segment vector d (R_prev x d = R_prev x R), free of the
 // cancellation of R_prev x R.  Zero rule: 1 - cos < 1e-8 <=> sin^2 < 2e-8
 // here (cos > 0 in the far field), sin^2 = |R_prev x d|^2 u_prev^2 u^2.
 // Records where some lane's estimate lies within a relative 5e-4 of the
-// threshold are noted in a wave-uniform mask and re-decided after the pass
-// with the reference's FP64 expression, correcting the sum where the two
-// decisions differ.
+// threshold are noted in a wave-uniform mask (and, per lane, whether it was
+// in the band and what it decided) and re-decided after the pass with the
+// reference's FP64 expression, correcting the sum where the decisions differ.
 __device__ __forceinline__ void far_tile(const FarRec* __restrict__ rec,
                                          const NearRec* __restrict__ nrec, const SrcTile& st,
                                          int base, const float off[3], const double rs[3],
-                                         float acc[3]) {
+                                         double acc[3]) {
   const float R0[3] = {st.prev_loc[0] + off[0], st.prev_loc[1] + off[1], st.prev_loc[2] + off[2]};
   float Rp[3] = {R0[0], R0[1], R0[2]};
   float l2p = Rp[0] * Rp[0] + Rp[1] * Rp[1] + Rp[2] * Rp[2];
   float up = __builtin_amdgcn_rsqf(l2p);
-  uint64_t bandmask = 0;
+  uint64_t bandmask = 0, lane_band = 0, lane_zero = 0;
   for (int e0 = 0; e0 < TILE; e0 += 8) {
     FarRec q[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) q[k] = rec[base + e0 + k];
+    float a8[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       Pair32 o = pair32(Rp, l2p, up, q[k], off);
       float f = o.g < 0.f ? 0.f : o.f;
-      acc[0] = fmaf(f, o.c[0], acc[0]);
-      acc[1] = fmaf(f, o.c[1], acc[1]);
-      acc[2] = fmaf(f, o.c[2], acc[2]);
-      if (__builtin_amdgcn_ballot_w64(fabsf(o.g) < 1e-11f)) bandmask |= 1ull << (e0 + k);
+      a8[0] = fmaf(f, o.c[0], a8[0]);
+      a8[1] = fmaf(f, o.c[1], a8[1]);
+      a8[2] = fmaf(f, o.c[2], a8[2]);
+      const bool band = fabsf(o.g) < 1e-11f;
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(band) != 0, 0)) {
+        bandmask |= 1ull << (e0 + k);
+        if (band) lane_band |= 1ull << (e0 + k);
+        if (band && o.g < 0.f) lane_zero |= 1ull << (e0 + k);
+      }
       Rp[0] = o.R[0];
       Rp[1] = o.R[1];
       Rp[2] = o.R[2];
       l2p = o.l2;
       up = o.u;
     }
+    acc[0] += (double)a8[0];
+    acc[1] += (double)a8[1];
+    acc[2] += (double)a8[2];
   }
   while (__builtin_expect(bandmask != 0, 0)) {
     const int e = __builtin_ctzll(bandmask);
@@ -566,15 +636,15 @@ __device__ __forceinline__ void far_tile(const FarRec* __restrict__ rec,
     float pl2 = P[0] * P[0] + P[1] * P[1] + P[2] * P[2];
     float pu = __builtin_amdgcn_rsqf(pl2);
     Pair32 o = pair32(P, pl2, pu, rec[base + e], off);
-    if (fabsf(o.g) < 1e-11f) {
-      const bool zero32 = o.g < 0.f;
+    if ((lane_band >> e) & 1) {
+      const bool zero32 = (lane_zero >> e) & 1;
       const double* a = e > 0 ? nrec[base + e - 1].p : st.prev;
       const bool zero = guard_exact(a, nrec[base + e].p, rs);
       if (zero != zero32) {
-        const float f = zero ? -o.f : o.f;
-        acc[0] = fmaf(f, o.c[0], acc[0]);
-        acc[1] = fmaf(f, o.c[1], acc[1]);
-        acc[2] = fmaf(f, o.c[2], acc[2]);
+        const double f = zero ? -(double)o.f : (double)o.f;
+        acc[0] += f * (double)o.c[0];
+        acc[1] += f * (double)o.c[1];
+        acc[2] += f * (double)o.c[2];
       }
     }
   }
@@ -630,13 +700,10 @@ __global__ void __launch_bounds__(256) k_direct(DirectArgs A) {
     }
     double v[3] = {0, 0, 0};
     if (A.mixed && d2 >= A.rnear2) {
-      float off[3], a32[3] = {0.f, 0.f, 0.f};
+      float off[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) off[i] = (float)(sj.c[i] - rs[i]);
-      far_tile(A.far, A.near, sj, J * TILE, off, rs, a32);
-      v[0] = (double)a32[0];
-      v[1] = (double)a32[1];
-      v[2] = (double)a32[2];
+      far_tile(A.far, A.near, sj, J * TILE, off, rs, v);
     } else {
       near_tile(A.near, sj, J * TILE, rs, s, im.primary != 0, v);
     }
