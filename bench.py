@@ -96,7 +96,7 @@ def main():
     wall = time.time() - wall0
     dev_ms = ctx.last_device_ms()
     I_eval = ctx.last_interactions()
-    n_launch, kd_ms = ctx.kernel_stats("k_direct")
+    n_launch, kd_ms = ctx.kernel_stats("k_direct_mixed")
     ms_per_step = dev_ms / args.steps
     value = 4 * I_eval * args.steps / (dev_ms / 1e3)
     log(f"timed: {args.steps} rk4 steps, device {dev_ms:.1f} ms, wall {wall:.1f} s, "
@@ -105,7 +105,7 @@ def main():
     achieved = FLOP_PER_INTERACTION * I_eval / kd_avg_s
     kernels = {}
     for k in ("k_geometry", "k_cont", "k_scan_blocks", "k_records", "k_src_tiles", "k_tgt_tiles",
-              "k_direct", "k_reduce", "k_epilogue", "k_stage", "k_combine"):
+              "k_adjacent", "k_direct_mixed", "k_reduce", "k_epilogue", "k_stage", "k_combine"):
         nl, ms = ctx.kernel_stats(k)
         if nl:
             kernels[k] = {"launches": nl, "avg_ms": round(ms / nl, 4)}
@@ -121,7 +121,7 @@ def main():
         ctx.reset_stats()
         ctx.velocities(b, 0.0, p_64, profile=True)
         f64_ms = ctx.last_device_ms()
-        nl64, kd64 = ctx.kernel_stats("k_direct")
+        nl64, kd64 = ctx.kernel_stats("k_direct_fp64")
         act = tg.active
         out["fp64_eval_ms"] = round(f64_ms, 2)
         out["mixed_eval_ms"] = round(mx_ms, 2)
@@ -131,7 +131,7 @@ def main():
         out["err_vels_mixed_vs_fp64"] = err_stats(a.vels[act], b.vels[act])
         out["err_vs_mixed_vs_fp64"] = err_stats(a.vs[act], b.vs[act])
         out["fp64_roofline"] = {
-            "bound": "valu", "achieved": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / 1e12, 3),
+            "kernel": "k_direct_fp64", "bound": "valu", "achieved": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / 1e12, 3),
             "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / PEAK_FP64, 4)}
         log(f"fp64 eval {f64_ms:.1f} ms, mixed eval {mx_ms:.1f} ms, "
@@ -185,7 +185,7 @@ def main():
         "rk4_steps_per_s": 1e3 / ms_per_step,
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4), "traffic": None,
-                     "kernel": "k_direct", "avg_launch_ms": round(kd_avg_s * 1e3, 2),
+                     "kernel": "k_direct_mixed", "avg_launch_ms": round(kd_avg_s * 1e3, 2),
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "cpu_baseline": cpu,
         "kernels": kernels,

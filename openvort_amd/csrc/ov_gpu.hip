@@ -60,7 +60,7 @@ struct ov_ctx {
   double *vs = nullptr, *vel_tmp = nullptr;
   double *T1 = nullptr, *N1 = nullptr, *dxi1 = nullptr, *vs1 = nullptr;
   int *fwd = nullptr, *rev = nullptr, *flag = nullptr;
-  int *brk = nullptr, *rank = nullptr, *bsum = nullptr, *counts = nullptr;
+  int *brk = nullptr, *rank = nullptr, *bsum = nullptr, *counts = nullptr, *adj = nullptr;
   TgtTile* tgt = nullptr;
   SrcTile* src = nullptr;
   FarRec* far = nullptr;
@@ -180,6 +180,7 @@ int ensure_capacity(ov_ctx* ctx, long need) {
   if ((rc = dalloc(ctx, (void**)&ctx->rank, sizeof(int) * cap))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->bsum, sizeof(int) * (cap / 256 + 1)))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->counts, sizeof(int) * 4))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->adj, sizeof(int) * 2 * cap))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->tgt, sizeof(TgtTile) * (cap / TILE)))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->src, sizeof(SrcTile) * (max_rec / TILE)))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->far, sizeof(FarRec) * max_rec))) return rc;
@@ -239,6 +240,8 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
         return fail(ctx, OV_E_UNSUPPORTED, "point %d neighbours pinned point %d", k, nb);
       if (sn != OV_FREE) return fail(ctx, OV_E_INVALID, "point %d links to empty point %d", k, nb);
     }
+    if (t->connections[f].reverse != k)
+      return fail(ctx, OV_E_INVALID, "links of %d and %d disagree", k, f);
     cid[k] = (int)ctx->act.size();
     ctx->act.push_back(k);
   }
@@ -423,6 +426,9 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
       return rc;
     if ((rc = launch(ctx, "k_tgt_tiles", k_tgt_tiles, dim3(n_tgt), dim3(TILE), X, n, cap, ctx->tgt)))
       return rc;
+    if ((rc = launch(ctx, "k_adjacent", k_adjacent, dim3((unsigned)((2 * n + 255) / 256)), dim3(256),
+                     (const NearRec*)ctx->near, (const int*)ctx->counts, ctx->adj)))
+      return rc;
     // enough (target tile, chunk) waves to fill 256 CUs several times over
     long chunks = (65536 + n_tgt - 1) / n_tgt;
     chunks = std::max<long>(1, std::min<long>(chunks, 256));
@@ -442,15 +448,18 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
     A.far = ctx->far;
     A.near = ctx->near;
     A.img = ctx->img;
+    A.adj = ctx->adj;
     A.n_img = n_img;
-    A.mixed = ctx->mixed;
     A.rnear2 = ctx->rnear * ctx->rnear;
     A.kappa4pi = ctx->ph.kappa / 4 / 3.14159265358979323846;
     A.partial = ctx->partial;
     A.n_chunks = (int)chunks;
-    if ((rc = launch(ctx, "k_direct", k_direct, dim3((unsigned)((n_tgt + 3) / 4), (unsigned)chunks),
-                     dim3(256), A)))
-      return rc;
+    const dim3 grid((unsigned)((n_tgt + 3) / 4), (unsigned)chunks);
+    if (ctx->mixed)
+      rc = launch(ctx, "k_direct_mixed", k_direct_mixed, grid, dim3(256), A);
+    else
+      rc = launch(ctx, "k_direct_fp64", k_direct_fp64, grid, dim3(256), A);
+    if (rc) return rc;
     if ((rc = launch(ctx, "k_reduce", k_reduce, grid1(n), dim3(256), (const double*)ctx->partial,
                      (int)chunks, n, cap, ctx->bs)))
       return rc;
@@ -562,7 +571,7 @@ void ov_ctx_destroy(ov_ctx* ctx) {
                   &ctx->vs1, &ctx->dxi, &ctx->dxi1, &ctx->partial};
   for (double** b : v)
     if (*b) hipFree(*b);
-  void* o[] = {ctx->fwd,  ctx->rev, ctx->flag, ctx->brk, ctx->rank, ctx->bsum, ctx->counts,
+  void* o[] = {ctx->fwd,  ctx->rev, ctx->flag, ctx->brk, ctx->rank, ctx->bsum, ctx->counts, ctx->adj,
                ctx->tgt,  ctx->src, ctx->far,  ctx->near, ctx->img};
   for (void* b : o)
     if (b) hipFree(b);

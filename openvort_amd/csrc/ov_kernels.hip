@@ -391,6 +391,21 @@ __global__ void __launch_bounds__(64) k_tgt_tiles(const double* __restrict__ P, 
   if (threadIdx.x == 0) tiles[T] = t;
 }
 
+// For each target, the records closing its two adjacent segments (the one
+// that starts at it and the one that ends at it, reference tangle.c:612-615):
+// adj[2 s] and adj[2 s + 1].  A wrapped adjacent segment lies a box length
+// away in the primary frame, so the sum must find it wherever it is.
+__global__ void __launch_bounds__(256) k_adjacent(const NearRec* __restrict__ near,
+                                                  const int* __restrict__ counts,
+                                                  int* __restrict__ adj) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= counts[0]) return;
+  const NearRec r = near[e];
+  if (r.seg < 0) return;
+  adj[2 * r.seg] = e;
+  adj[2 * r.seg_fwd + 1] = e;
+}
+
 // ---------------------------------------------------------------------------
 // direct Biot-Savart sum over all segments and images
 // ---------------------------------------------------------------------------
@@ -438,9 +453,9 @@ struct DirectArgs {
   const FarRec* far;
   const NearRec* near;
   const ImgDev* img;
+  const int* adj;   // [2 n] records of each target's adjacent segments
   int n_img;
-  int mixed;        // 1: FP32 far field beyond rnear2
-  double rnear2;    // squared box distance of the FP64 near field
+  double rnear2;    // squared box distance of the FP64 near field (mixed)
   double kappa4pi;
   double* partial;  // [n_chunks][3][cap]
   int n_chunks;
@@ -653,7 +668,9 @@ __device__ __forceinline__ void far_tile(const FarRec* __restrict__ rec,
 // Grid: blockIdx.x = group of 4 target tiles (one wavefront each),
 // blockIdx.y = chunk of the (source tile, image) work list.  Each wave writes
 // its chunk's sum to partial[chunk]; k_reduce adds the chunks in order.
-__global__ void __launch_bounds__(256) k_direct(DirectArgs A) {
+// MIXED: FP32 pass for tiles beyond the near radius; otherwise all FP64.
+template <bool MIXED>
+__device__ __forceinline__ void direct_body(const DirectArgs& A) {
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int T = blockIdx.x * 4 + wid;
@@ -671,6 +688,8 @@ __global__ void __launch_bounds__(256) k_direct(DirectArgs A) {
   } else {
     load3(A.P, A.cap, T * TILE, r);
   }
+  const int adjA = valid ? A.adj[2 * s] / TILE : -1;
+  const int adjB = valid ? A.adj[2 * s + 1] / TILE : -1;
   double acc[3] = {0, 0, 0};
   for (long w = w0; w < w1; ++w) {
     const int J = (int)(w / A.n_img);
@@ -699,7 +718,10 @@ __global__ void __launch_bounds__(256) k_direct(DirectArgs A) {
       d2 += g * g;
     }
     double v[3] = {0, 0, 0};
-    if (A.mixed && d2 >= A.rnear2) {
+    // the primary sum skips each target's adjacent segments: a tile holding
+    // one (possibly wrapped, hence far away) goes through the FP64 pass
+    const bool adjacent = im.primary && __builtin_amdgcn_ballot_w64(J == adjA || J == adjB) != 0;
+    if (MIXED && d2 >= A.rnear2 && !adjacent) {
       float off[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) off[i] = (float)(sj.c[i] - rs[i]);
@@ -719,6 +741,9 @@ __global__ void __launch_bounds__(256) k_direct(DirectArgs A) {
     out[2 * A.cap + s] = acc[2] * A.kappa4pi;
   }
 }
+
+__global__ void __launch_bounds__(256) k_direct_mixed(DirectArgs A) { direct_body<true>(A); }
+__global__ void __launch_bounds__(256) k_direct_fp64(DirectArgs A) { direct_body<false>(A); }
 
 __global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ partial, int n_chunks,
                                                 int n, long cap, double* __restrict__ bs) {

@@ -223,3 +223,18 @@ def test_gpu_matches_golden(ctx, name):
         d = np.abs(x.vnodes[act] - z["out_vnodes_after"][act]).max()
         print(name, "positions after", steps, "steps:", d, flush=True)
         assert d < 1e-16 * max(1, steps)
+
+
+@pytest.mark.parametrize("rnear", [0.005, 0.02])
+def test_mixed_skips_wrapped_adjacent_segment(ctx, oracle, rnear):
+    """A ring crossing the x faces of a periodic box: at the crossing the
+    segment into a target wraps, so in the primary frame it lies one box
+    length away, in an FP32 tile; the primary sum must still skip it."""
+    b0 = T.TangleBuilder()
+    b0.add_ring((0.001, 0.05, 0.05), (0.0, 0.0, 1.0), 0.01, 300)
+    t = b0.build((0, 0, 0), (0.1, 0.1, 0.1), (T.WALL_PERIODIC,) * 6, T.periodic_images(26))
+    t.wrap_into_box()
+    p = tangle_params(precision=T.PREC_MIXED, near_radius=rnear)
+    a, b = _both(t, lambda x: oracle.update_tangle(x, 0.0, p),
+                 lambda x: ctx.velocities(x, 0.0, p, geometry=True))
+    _check_vel(a, b, MIXED_TOL)
