@@ -64,8 +64,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "sharding of this path is described in DESIGN.md but not built", flush=True)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no "
+              "multi-GPU path (targets would be sharded by stream tiles with one all-gather "
+              "of positions per RK stage)", flush=True)
         sys.exit(2)
 
     from openvort_amd import tangle as T
@@ -80,8 +81,10 @@ def main():
     p_64 = T.default_params(precision=T.PREC_FP64, **common)
 
     ctx = Context(0)
-    # load the code objects on a tiny case (not a step of the workload)
-    ctx.velocities(T.ring(64), 0.0, p_mx)
+    # load the code objects on a tiny case (not a step of the workload; local
+    # induction only, so the profile of the direct-sum kernels holds only
+    # launches on the workload)
+    ctx.velocities(T.ring(64), 0.0, T.default_params(LIA_only=1, **common))
 
     for w in range(args.warmup):
         x = tg.copy()

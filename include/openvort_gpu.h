@@ -119,7 +119,7 @@ enum {
 /* Flags of ov_gpu_velocities / ov_gpu_rk4_step / ov_gpu_rk4_run. */
 enum {
   OV_UPDATE_GEOMETRY = 1,        /* recompute dxi/tangents/normals first (update_tangle) */
-  OV_USE_CALLER_K1 = 2,          /* rk4: k1 = tangle->vels as given (rk4_step alone, G16) */
+  OV_USE_CALLER_K1 = 2,          /* rk4: k1 = tangle->vels as given (rk4_step alone, vortex_dynamics.c:100) */
   OV_ENFORCE_BOUNDARIES = 4,     /* rk4_run: periodic wrap after each step (tangle.c:796-819) */
   OV_PROFILE = 8                 /* time each kernel with HIP events (ov_ctx_kernel_stats) */
 };

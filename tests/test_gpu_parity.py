@@ -119,7 +119,8 @@ def test_rk4_step_periodic(ctx, oracle, variant):
 
 
 def test_rk4_caller_k1(ctx, oracle):
-    """rk4_step alone: k1 is whatever the caller left in vels (G16)."""
+    """rk4_step alone: k1 is whatever the caller left in vels (reference
+    vortex_dynamics.c:100)."""
     t = T.random_rings(3, 0.1, n_rings=3)
     p = tangle_params()
     oracle.update_tangle(t, 0.0, p)
