@@ -17,6 +17,8 @@ the same positions, the mixed form's error against it, the roofline of the
 dominant kernel (k_direct) and a CPU baseline: the reference's own OpenMP
 update_velocity (oracle/_ref, or the FP64 restatement where that is absent)
 on a seeded sample of targets of the same tangle, on this host's cores.
+The Barnes-Hut octree (theta 0.3/0.5/0.7, FP64 and mixed) is evaluated on the
+same positions and compared with the FP64 direct sum on 4096 seeded targets.
 """
 from __future__ import annotations
 
@@ -61,6 +63,8 @@ def main():
     ap.add_argument("--fp64", type=int, default=1, help="FP64 evaluations for the comparison (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=64, help="targets of the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--tree", type=int, default=1,
+                    help="octree evaluations at theta 0.3/0.5/0.7 against the FP64 direct sum (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -139,6 +143,28 @@ def main():
             "frac": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / PEAK_FP64, 4)}
         log(f"fp64 eval {f64_ms:.1f} ms, mixed eval {mx_ms:.1f} ms, "
             f"err {out['err_vels_mixed_vs_fp64']}")
+
+    # ---- Barnes-Hut octree vs the FP64 direct sum (seeded 4096-target sample)
+    if args.tree > 0 and args.fp64 > 0:
+        rng_t = np.random.default_rng(2002)
+        samp = np.sort(rng_t.choice(tg.active, size=min(4096, n), replace=False))
+        tree = []
+        for prec, pname in ((T.PREC_FP64, "fp64"), (T.PREC_MIXED, "mixed")):
+            for th in (0.3, 0.5, 0.7):
+                x = tg.copy()
+                ctx.reset_stats()
+                ctx.tree_stats()
+                ctx.velocities(x, 0.0, T.default_params(precision=prec, use_BH=1, BH_resolution=th,
+                                                        **common), profile=True)
+                ms = ctx.last_device_ms()
+                cells, pairs, mps = ctx.tree_stats()
+                kname = "k_octree_fp64" if prec == T.PREC_FP64 else "k_octree_mixed"
+                tree.append({"theta": th, "precision": pname, "eval_ms": round(ms, 2),
+                             "traversal_ms": round(ctx.kernel_stats(kname)[1], 2),
+                             "cells": int(cells), "leaf_pairs": pairs, "multipoles": mps,
+                             "err_vels_vs_fp64_direct_sample": err_stats(x.vels[samp], b.vels[samp])})
+                log(f"tree {pname} theta {th}: {ms:.1f} ms, err {tree[-1]['err_vels_vs_fp64_direct_sample']}")
+        out["octree"] = tree
 
     # ---- CPU baseline: the reference's update_velocity on sampled targets ---
     cpu = None

@@ -172,6 +172,11 @@ double ov_ctx_last_interactions(const ov_ctx *ctx);
  * on the context stream around the kernels, host<->device copies excluded. */
 double ov_ctx_last_device_ms(const ov_ctx *ctx);
 
+/* Octree of the last evaluation with use_BH: cell count; leaf segment-point
+ * interactions and multipole evaluations summed over the evaluations made
+ * with OV_PROFILE since the previous call (then reset). */
+void ov_ctx_tree_stats(ov_ctx *ctx, double *cells, double *leaf_pairs, double *multipoles);
+
 #ifdef __cplusplus
 }
 #endif

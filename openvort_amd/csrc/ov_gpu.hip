@@ -19,6 +19,9 @@
 #include "../../include/openvort_gpu.h"
 #include "ov_device.h"
 #include "ov_kernels.hip"
+#include "ov_octree.hip"
+
+#include <hipcub/hipcub.hpp>
 
 using namespace ov;
 
@@ -80,6 +83,26 @@ struct ov_ctx {
   std::vector<hipEvent_t> ev_pool;
   std::map<std::string, KStat> stats;
   double last_interactions = 0;
+  // octree
+  bool use_tree = false;
+  OctParams op{};
+  std::vector<int> pos_by_index;   // ascending tangle index -> stream index
+  int* t_order = nullptr;
+  uint64_t *t_k[3] = {nullptr, nullptr, nullptr}, *t_ks[3] = {nullptr, nullptr, nullptr};
+  uint64_t *t_ktmp = nullptr, *t_kout = nullptr;
+  int *t_vals = nullptr, *t_vals2 = nullptr, *t_sorted = nullptr, *t_maxd = nullptr;
+  LeafSeg* t_segs = nullptr;
+  long t_pcap = 0;
+  OctCell* t_cells = nullptr;
+  int *t_nchild = nullptr, *t_offs = nullptr, *t_size = nullptr, *t_pre = nullptr;
+  double* t_mom = nullptr;
+  OctNode* t_nodes = nullptr;
+  long t_ccap = 0;
+  void* t_cub = nullptr;
+  size_t t_cub_bytes = 0;
+  long* t_stats = nullptr;
+  long tree_cells = 0;
+  double tree_leaf_pairs = 0, tree_multipoles = 0;
   hipEvent_t t_begin = nullptr, t_end = nullptr;
   double last_device_ms = 0;
 };
@@ -215,7 +238,6 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   if (p->vs.type < OV_FLOW_NONE || p->vs.type > OV_FLOW_SIMPLE || p->vn.type < OV_FLOW_NONE ||
       p->vn.type > OV_FLOW_SIMPLE)
     return fail(ctx, OV_E_UNSUPPORTED, "only the 'no flow' and 'simple' flows run on the device");
-  if (p->use_BH) return fail(ctx, OV_E_UNSUPPORTED, "octree path not available in this build");
   if (t->bimg.n < 0 || t->bimg.n > MAX_IMAGES - 1 || (t->bimg.n > 0 && !t->bimg.images))
     return fail(ctx, OV_E_INVALID, "bad image list (n=%d)", t->bimg.n);
   const int N = t->N;
@@ -260,6 +282,7 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   }
   std::vector<int> pos(n);
   for (int q = 0; q < n; ++q) pos[order[q]] = q;
+  ctx->pos_by_index = pos;   // compact ids are in ascending tangle index
   std::vector<int> act_s(n);
   ctx->fwd_s.assign(n, 0);
   ctx->rev_s.assign(n, 0);
@@ -322,6 +345,13 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   ph.hyperfriction = p->hyperfriction;
   ctx->lia_only = p->LIA_only != 0;
   ctx->mixed = p->precision == OV_PREC_MIXED;
+  ctx->use_tree = p->use_BH != 0;
+  ctx->op.box = ctx->box;
+  ctx->op.grain = p->BH_grain;
+  ctx->op.grain_dl = p->BH_grain * p->dl_max;
+  ctx->op.theta = p->BH_resolution;
+  ctx->op.quad = p->BH_quadtree != 0;
+  ctx->op.dipole = p->multipole == OV_MP_DIPOLE;
   ctx->rnear = p->near_radius > 0 ? p->near_radius : 0.02;
   ctx->profile = false;
 
@@ -393,6 +423,225 @@ int upload_topology(ov_ctx* ctx) {
 
 inline dim3 grid1(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// ---- octree -----------------------------------------------------------------
+
+int tree_point_buffers(ov_ctx* ctx, long need) {
+  if (need <= ctx->t_pcap) return OV_OK;
+  long cap = std::max<long>(need, 2 * ctx->t_pcap);
+  int rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_order, sizeof(int) * cap))) return rc;
+  for (int w = 0; w < 3; ++w) {
+    if ((rc = dalloc(ctx, (void**)&ctx->t_k[w], sizeof(uint64_t) * cap))) return rc;
+    if ((rc = dalloc(ctx, (void**)&ctx->t_ks[w], sizeof(uint64_t) * cap))) return rc;
+  }
+  if ((rc = dalloc(ctx, (void**)&ctx->t_ktmp, sizeof(uint64_t) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_kout, sizeof(uint64_t) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_vals, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_vals2, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_sorted, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_segs, sizeof(LeafSeg) * cap))) return rc;
+  if (!ctx->t_maxd && (rc = dalloc(ctx, (void**)&ctx->t_maxd, 256))) return rc;
+  if (!ctx->t_stats && (rc = dalloc(ctx, (void**)&ctx->t_stats, 256))) return rc;
+  ctx->t_pcap = cap;
+  return OV_OK;
+}
+
+// Cell buffers for `need` cells; existing cells are kept.
+int tree_cell_buffers(ov_ctx* ctx, long need, long keep) {
+  if (need <= ctx->t_ccap) return OV_OK;
+  long cap = std::max<long>(need, 2 * ctx->t_ccap);
+  OctCell* nc = nullptr;
+  if (hipMalloc((void**)&nc, sizeof(OctCell) * cap) != hipSuccess)
+    return fail(ctx, OV_E_NOMEM, "hipMalloc(%ld cells)", cap);
+  if (ctx->t_cells && keep > 0)
+    HIPCHK(hipMemcpyAsync(nc, ctx->t_cells, sizeof(OctCell) * keep, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->t_cells) hipFree(ctx->t_cells);
+  ctx->t_cells = nc;
+  int rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_nchild, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_offs, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_size, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_pre, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_mom, sizeof(double) * 15 * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_nodes, sizeof(OctNode) * cap))) return rc;
+  ctx->t_ccap = cap;
+  return OV_OK;
+}
+
+int cub_temp(ov_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->t_cub_bytes) return OV_OK;
+  int rc = dalloc(ctx, &ctx->t_cub, bytes);
+  if (rc) return rc;
+  ctx->t_cub_bytes = bytes;
+  return OV_OK;
+}
+
+// Build the reference's octree of the points X (stream order) on the device.
+int build_tree(ov_ctx* ctx, const double* X) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  int rc;
+  if ((rc = tree_point_buffers(ctx, n))) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->t_order, ctx->pos_by_index.data(), sizeof(int) * n,
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->t_maxd, 0, sizeof(int), ctx->stream));
+  const dim3 gp((unsigned)((n + 255) / 256));
+  if ((rc = launch(ctx, "k_oct_keys", k_oct_keys, gp, dim3(256), X, cap, (const int*)ctx->t_order, n,
+                   ctx->op, ctx->t_k[0], ctx->t_k[1], ctx->t_k[2], ctx->t_vals, ctx->t_maxd)))
+    return rc;
+  int maxd = 0;
+  HIPCHK(hipMemcpyAsync(&maxd, ctx->t_maxd, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int words = std::max(1, (maxd + 15) / 16);
+  // stable LSD passes over the path words, least significant word first
+  size_t need = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, ctx->t_ktmp, ctx->t_kout, ctx->t_vals,
+                                            ctx->t_vals2, n, 0, 64, ctx->stream));
+  if ((rc = cub_temp(ctx, need))) return rc;
+  for (int w = words - 1; w >= 0; --w) {
+    if (w == words - 1) {
+      HIPCHK(hipMemcpyAsync(ctx->t_ktmp, ctx->t_k[w], sizeof(uint64_t) * n, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    } else if ((rc = launch(ctx, "k_gather_u64", k_gather_u64, gp, dim3(256),
+                            (const uint64_t*)ctx->t_k[w], (const int*)ctx->t_vals, n, ctx->t_ktmp))) {
+      return rc;
+    }
+    size_t tb = ctx->t_cub_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->t_cub, tb, ctx->t_ktmp, ctx->t_kout, ctx->t_vals,
+                                              ctx->t_vals2, n, 0, 64, ctx->stream));
+    std::swap(ctx->t_vals, ctx->t_vals2);
+  }
+  // sorted words and the sorted points' segments
+  for (int w = 0; w < 3; ++w) {
+    if (w < words) {
+      if ((rc = launch(ctx, "k_gather_u64", k_gather_u64, gp, dim3(256), (const uint64_t*)ctx->t_k[w],
+                       (const int*)ctx->t_vals, n, ctx->t_ks[w])))
+        return rc;
+    } else {
+      HIPCHK(hipMemsetAsync(ctx->t_ks[w], 0, sizeof(uint64_t) * n, ctx->stream));
+    }
+  }
+  if ((rc = launch(ctx, "k_oct_leafsegs", k_oct_leafsegs, gp, dim3(256), X, cap, (const int*)ctx->fwd,
+                   (const int*)ctx->t_order, (const int*)ctx->t_vals, n, ctx->box, ctx->t_sorted,
+                   ctx->t_segs)))
+    return rc;
+  // cells, level by level
+  if ((rc = tree_cell_buffers(ctx, 4L * n + 64, 0))) return rc;
+  OctCell root{};
+  for (int i = 0; i < 3; ++i) {
+    root.lo[i] = ctx->box.lo[i];
+    root.hi[i] = ctx->box.hi[i];
+  }
+  root.start = 0;
+  root.count = n;
+  root.level = 0;
+  root.parent = -1;
+  root.child_first = -1;
+  root.n_child = 0;
+  HIPCHK(hipMemcpyAsync(ctx->t_cells, &root, sizeof(OctCell), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<std::pair<int, int>> levels;
+  int ls = 0, le = 1;
+  while (le > ls) {
+    levels.push_back({ls, le});
+    const int nl = le - ls;
+    const dim3 gl((unsigned)((nl + 255) / 256));
+    if ((rc = launch(ctx, "k_oct_count", k_oct_count, gl, dim3(256), (const OctCell*)ctx->t_cells, ls, le,
+                     (const uint64_t*)ctx->t_ks[0], (const uint64_t*)ctx->t_ks[1],
+                     (const uint64_t*)ctx->t_ks[2], ctx->op, ctx->t_nchild)))
+      return rc;
+    size_t sb = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, ctx->t_nchild, ctx->t_offs, nl, ctx->stream));
+    if ((rc = cub_temp(ctx, sb))) return rc;
+    sb = ctx->t_cub_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->t_cub, sb, ctx->t_nchild, ctx->t_offs, nl, ctx->stream));
+    int tail[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&tail[0], ctx->t_offs + nl - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&tail[1], ctx->t_nchild + nl - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int total = tail[0] + tail[1];
+    if ((rc = tree_cell_buffers(ctx, (long)le + total + 1, le))) return rc;
+    if ((rc = launch(ctx, "k_oct_emit", k_oct_emit, gl, dim3(256), ctx->t_cells, ls, le, le,
+                     (const uint64_t*)ctx->t_ks[0], (const uint64_t*)ctx->t_ks[1],
+                     (const uint64_t*)ctx->t_ks[2], ctx->op, (const int*)ctx->t_offs)))
+      return rc;
+    ls = le;
+    le = le + total;
+  }
+  const int ncell = le;
+  ctx->tree_cells = ncell;
+  if ((rc = launch(ctx, "k_oct_moments", k_oct_moments, dim3((unsigned)((ncell + 3) / 4)), dim3(256),
+                   (const OctCell*)ctx->t_cells, ncell, (const LeafSeg*)ctx->t_segs, ctx->t_mom)))
+    return rc;
+  for (int L = (int)levels.size() - 1; L >= 0; --L) {
+    const int a = levels[L].first, b = levels[L].second;
+    if ((rc = launch(ctx, "k_oct_size", k_oct_size, dim3((unsigned)((b - a + 255) / 256)), dim3(256),
+                     (const OctCell*)ctx->t_cells, a, b, ctx->t_size)))
+      return rc;
+  }
+  HIPCHK(hipMemsetAsync(ctx->t_pre, 0, sizeof(int), ctx->stream));
+  for (size_t L = 0; L < levels.size(); ++L) {
+    const int a = levels[L].first, b = levels[L].second;
+    if ((rc = launch(ctx, "k_oct_pre", k_oct_pre, dim3((unsigned)((b - a + 255) / 256)), dim3(256),
+                     (const OctCell*)ctx->t_cells, a, b, (const int*)ctx->t_size, ctx->t_pre)))
+      return rc;
+  }
+  if ((rc = launch(ctx, "k_oct_nodes", k_oct_nodes, dim3((unsigned)((ncell + 255) / 256)), dim3(256),
+                   (const OctCell*)ctx->t_cells, ncell, (const double*)ctx->t_mom,
+                   (const int*)ctx->t_size, (const int*)ctx->t_pre, ctx->t_nodes)))
+    return rc;
+  return OV_OK;
+}
+
+// Barnes-Hut sum over the tree of X for every target and image into ctx->bs.
+int tree_sum(ov_ctx* ctx, const double* X) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  int rc;
+  if ((rc = build_tree(ctx, X))) return rc;
+  const int n_tgt = (n + TILE - 1) / TILE;
+  const int n_img = (int)ctx->imgs.size();
+  size_t pbytes = sizeof(double) * 3 * (size_t)cap * (size_t)n_img;
+  if (pbytes > ctx->partial_bytes) {
+    if ((rc = dalloc(ctx, (void**)&ctx->partial, pbytes))) return rc;
+    ctx->partial_bytes = pbytes;
+  }
+  if ((rc = launch(ctx, "k_tgt_tiles", k_tgt_tiles, dim3(n_tgt), dim3(TILE), X, n, cap, ctx->tgt)))
+    return rc;
+  HIPCHK(hipMemsetAsync(ctx->t_stats, 0, 3 * sizeof(long), ctx->stream));
+  OctArgs A;
+  A.P = X;
+  A.cap = cap;
+  A.n = n;
+  A.n_tgt_tiles = n_tgt;
+  A.nodes = ctx->t_nodes;
+  A.n_nodes = (int)ctx->tree_cells;
+  A.segs = ctx->t_segs;
+  A.img = ctx->img;
+  A.n_img = n_img;
+  A.op = ctx->op;
+  A.kappa4pi = ctx->ph.kappa / 4 / 3.14159265358979323846;
+  A.partial = ctx->partial;
+  A.stats = ctx->profile ? ctx->t_stats : nullptr;
+  const dim3 grid((unsigned)((n_tgt + 3) / 4), (unsigned)n_img);
+  if (ctx->mixed)
+    rc = launch(ctx, "k_octree_mixed", k_octree_mixed, grid, dim3(256), A);
+  else
+    rc = launch(ctx, "k_octree_fp64", k_octree_fp64, grid, dim3(256), A);
+  if (rc) return rc;
+  if ((rc = launch(ctx, "k_reduce", k_reduce, grid1(n), dim3(256), (const double*)ctx->partial, n_img, n,
+                   cap, ctx->bs)))
+    return rc;
+  if (ctx->profile) {
+    long st[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(st, ctx->t_stats, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->tree_leaf_pairs += (double)st[0];
+    ctx->tree_multipoles += (double)st[1];
+  }
+  return OV_OK;
+}
+
 // One velocity evaluation at positions X: geometry (or the caller's s', s''
 // already in Tg/Ng), the direct sum, the epilogue.
 int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, double* dxi,
@@ -406,7 +655,11 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
                    ctx->lia)))
     return rc;
   int have_bs = 0;
-  if (!ctx->lia_only) {
+  if (!ctx->lia_only && ctx->use_tree) {
+    if ((rc = tree_sum(ctx, X))) return rc;
+    have_bs = 1;
+    ctx->last_interactions = 0;
+  } else if (!ctx->lia_only) {
     const int n_tgt = (n + TILE - 1) / TILE;
     const int nb = (n + 255) / 256;
     const int max_src = (2 * n + TILE - 1) / TILE;
@@ -575,6 +828,12 @@ void ov_ctx_destroy(ov_ctx* ctx) {
                ctx->tgt,  ctx->src, ctx->far,  ctx->near, ctx->img};
   for (void* b : o)
     if (b) hipFree(b);
+  void* tb[] = {ctx->t_order, ctx->t_k[0], ctx->t_k[1], ctx->t_k[2], ctx->t_ks[0], ctx->t_ks[1],
+                ctx->t_ks[2], ctx->t_ktmp, ctx->t_kout, ctx->t_vals, ctx->t_vals2, ctx->t_sorted,
+                ctx->t_maxd, ctx->t_segs, ctx->t_cells, ctx->t_nchild, ctx->t_offs, ctx->t_size,
+                ctx->t_pre, ctx->t_mom, ctx->t_nodes, ctx->t_cub, ctx->t_stats};
+  for (void* b : tb)
+    if (b) hipFree(b);
   if (ctx->hst) hipHostFree(ctx->hst);
   for (auto& p : ctx->pending) {
     hipEventDestroy(p.a);
@@ -609,6 +868,15 @@ void ov_ctx_reset_stats(ov_ctx* ctx) {
 double ov_ctx_last_interactions(const ov_ctx* ctx) { return ctx ? ctx->last_interactions : 0; }
 
 double ov_ctx_last_device_ms(const ov_ctx* ctx) { return ctx ? ctx->last_device_ms : 0; }
+
+void ov_ctx_tree_stats(ov_ctx* ctx, double* cells, double* leaf_pairs, double* multipoles) {
+  if (!ctx) return;
+  if (cells) *cells = (double)ctx->tree_cells;
+  if (leaf_pairs) *leaf_pairs = ctx->tree_leaf_pairs;
+  if (multipoles) *multipoles = ctx->tree_multipoles;
+  ctx->tree_leaf_pairs = 0;
+  ctx->tree_multipoles = 0;
+}
 
 int ov_gpu_update_tangents_normals(ov_ctx* ctx, ov_tangle_state* t) {
   if (!ctx) return OV_E_INVALID;

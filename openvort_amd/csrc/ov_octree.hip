@@ -1,0 +1,550 @@
+// ov_octree.hip — Barnes-Hut octree on the device, cell for cell the tree of
+// the reference (octree.c:79-343).
+//
+// Build:
+//  1. k_oct_keys: every point descends from the domain box, computing each
+//     child box with the reference's arithmetic (octree.c:158-235) and taking
+//     the first child, in child order, that contains it strictly
+//     (vec3_maths.c:67-77); the child indices form a path of 4-bit digits
+//     (8 = contained in no child: dropped at that level, octree.c:118-127).
+//  2. Paths are sorted stably (points enter in ascending tangle index), so
+//     every cell is a contiguous range.
+//  3. Cells are built level by level: a cell with more than one point whose
+//     shortest edge is >= BH_grain * dl_max (octree.c:99-108) gets one child
+//     per non-empty digit group.
+//  4. Moments per cell (octree.c:237-266): centre = (sum r) / n, circulation
+//     Gamma = sum ds, tensor M_ij = sum ds_i (r - centre)_j (one wavefront per
+//     cell; summation order differs from the reference's by rounding only).
+//  5. Cells are renumbered in preorder; `skip` = subtree size.
+// Traversal (octree.c:275-343), one wavefront per 64 targets and image: the
+// wave walks nodes in preorder; a lane that takes a node (empty, leaf or
+// accepted multipole) waits until the wave leaves that subtree; the wave
+// descends when any active lane opens the node and otherwise jumps over it.
+#include <hip/hip_runtime.h>
+
+#include "ov_device.h"
+
+namespace ov {
+
+constexpr int OCT_MAXD = 48;       // path digits (3 x 16 nibbles)
+
+struct OctCell {                   // build-time cell, breadth-first order
+  double lo[3], hi[3];
+  int start, count;                // range in the sorted point order
+  int level, parent;
+  int child_first, n_child;
+};
+
+struct OctNode {                   // traversal node, preorder
+  double com[3];
+  double gam[3];
+  double M[9];
+  double lmax, lmin;
+  int n, first, skip, pad;
+};
+
+struct LeafSeg {                   // segment of a sorted point
+  double a[3], b[3];
+  int m, fwd;
+};
+
+struct OctParams {
+  BoxDev box;
+  double grain;                    // BH_grain (leaf test, octree.c:289)
+  double grain_dl;                 // BH_grain * dl_max (split test, octree.c:104)
+  double theta;                    // BH_resolution
+  int quad;
+  int dipole;                      // 1: true dipole instead of the reference form
+};
+
+// Child-box faces (octree.c:158-235): per child (BLF, BLB, BRF, BRB, TLF,
+// TLB, TRF, TRB) and axis, the low face is one of {P.lo, P.lo + L/2, c,
+// c - L/2} and the high face one of {c, c + L/2, P.hi, P.hi - L/2}.
+__constant__ unsigned char oct_lo_code[8][3] = {
+    {1, 0, 0}, {0, 0, 0}, {2, 2, 3}, {0, 1, 0}, {2, 3, 2}, {0, 0, 1}, {2, 2, 2}, {3, 2, 2}};
+__constant__ unsigned char oct_hi_code[8][3] = {
+    {1, 0, 0}, {0, 0, 0}, {2, 2, 3}, {0, 1, 0}, {2, 3, 2}, {0, 0, 1}, {2, 2, 2}, {3, 2, 2}};
+
+// (lo, hi may alias plo, phi.)
+__device__ inline void oct_child_box(const double plo[3], const double phi[3], int ci, int quad,
+                                     double lo[3], double hi[3]) {
+#pragma clang fp contract(off)
+  double nlo[3], nhi[3];
+  for (int i = 0; i < 3; ++i) {
+    double c = plo[i] + phi[i];
+    c = c * 0.5;
+    double L = phi[i] - plo[i];
+    switch (oct_lo_code[ci][i]) {
+      case 0: nlo[i] = plo[i]; break;
+      case 1: nlo[i] = plo[i] + L / 2; break;
+      case 2: nlo[i] = c; break;
+      default: nlo[i] = c - L / 2; break;
+    }
+    switch (oct_hi_code[ci][i]) {
+      case 0: nhi[i] = c; break;
+      case 1: nhi[i] = c + L / 2; break;
+      case 2: nhi[i] = phi[i]; break;
+      default: nhi[i] = phi[i] - L / 2; break;
+    }
+  }
+  if (quad) nhi[2] = phi[2];
+  for (int i = 0; i < 3; ++i) {
+    lo[i] = nlo[i];
+    hi[i] = nhi[i];
+  }
+}
+
+__device__ inline double oct_min_len(const double lo[3], const double hi[3]) {
+  double Lx = hi[0] - lo[0], Ly = hi[1] - lo[1], Lz = hi[2] - lo[2];
+  if (Lx < Ly) return Lx < Lz ? Lx : Lz;
+  return Ly < Lz ? Ly : Lz;
+}
+__device__ inline double oct_max_len(const double lo[3], const double hi[3]) {
+  double Lx = hi[0] - lo[0], Ly = hi[1] - lo[1], Lz = hi[2] - lo[2];
+  if (Lx > Ly) return Lx > Lz ? Lx : Lz;
+  return Ly > Lz ? Ly : Lz;
+}
+
+__device__ inline int oct_digit(const uint64_t* __restrict__ k0, const uint64_t* __restrict__ k1,
+                                const uint64_t* __restrict__ k2, int j, int level0) {
+  // digit of level level0 + 1 (0-based nibble index level0)
+  const int w = level0 >> 4, sh = 60 - 4 * (level0 & 15);
+  const uint64_t v = w == 0 ? k0[j] : (w == 1 ? k1[j] : k2[j]);
+  return (int)((v >> sh) & 15);
+}
+
+// 1. Path of every point.  Point i of the ascending-tangle-index order is
+// stream point order[i].
+__global__ void __launch_bounds__(256) k_oct_keys(const double* __restrict__ P, long cap,
+                                                  const int* __restrict__ order, int n,
+                                                  OctParams op, uint64_t* __restrict__ k0,
+                                                  uint64_t* __restrict__ k1,
+                                                  uint64_t* __restrict__ k2,
+                                                  int* __restrict__ vals, int* __restrict__ maxdepth) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int s = order[i];
+  const double r[3] = {P[s], P[cap + s], P[2 * cap + s]};
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = op.box.lo[a];
+    hi[a] = op.box.hi[a];
+  }
+  uint64_t w[3] = {0, 0, 0};
+  const int nch = op.quad ? 4 : 8;
+  int depth = 0;
+  while (depth < OCT_MAXD && !(oct_min_len(lo, hi) < op.grain_dl)) {
+    int found = 8;
+    double clo[3], chi[3];
+    for (int ci = 0; ci < nch; ++ci) {
+      oct_child_box(lo, hi, ci, op.quad, clo, chi);
+      if (r[0] > clo[0] && r[0] < chi[0] && r[1] > clo[1] && r[1] < chi[1] && r[2] > clo[2] &&
+          r[2] < chi[2]) {
+        found = ci;
+        break;
+      }
+    }
+    w[depth >> 4] |= (uint64_t)found << (60 - 4 * (depth & 15));
+    ++depth;
+    if (found == 8) break;
+    oct_child_box(lo, hi, found, op.quad, lo, hi);
+  }
+  k0[i] = w[0];
+  k1[i] = w[1];
+  k2[i] = w[2];
+  vals[i] = i;
+  atomicMax(maxdepth, depth);
+}
+
+__global__ void __launch_bounds__(256) k_gather_u64(const uint64_t* __restrict__ src,
+                                                    const int* __restrict__ idx, int n,
+                                                    uint64_t* __restrict__ dst) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// Sorted position j -> stream point, and the point's segment (the segment
+// of octree cells, octree.c:253-258 and tangle.c:519-528).
+__global__ void __launch_bounds__(256) k_oct_leafsegs(const double* __restrict__ P, long cap,
+                                                      const int* __restrict__ fwd,
+                                                      const int* __restrict__ order,
+                                                      const int* __restrict__ perm, int n,
+                                                      BoxDev box, int* __restrict__ sorted_s,
+                                                      LeafSeg* __restrict__ segs) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int s = order[perm[j]];
+  sorted_s[j] = s;
+  LeafSeg q;
+  const int f = fwd[s];
+  const double pf[3] = {P[f], P[cap + f], P[2 * cap + f]};
+  q.a[0] = P[s];
+  q.a[1] = P[cap + s];
+  q.a[2] = P[2 * cap + s];
+  pwrap(q.a, pf, box, q.b);
+  q.m = s;
+  q.fwd = f;
+  segs[j] = q;
+}
+
+// First position in [lo, hi) whose digit (level0) is >= d.
+__device__ inline int oct_lower_bound(const uint64_t* k0, const uint64_t* k1, const uint64_t* k2,
+                                      int lo, int hi, int level0, int d) {
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (oct_digit(k0, k1, k2, mid, level0) < d)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline bool oct_splits(const OctCell& c, const OctParams& op) {
+  return c.count > 1 && !(oct_min_len(c.lo, c.hi) < op.grain_dl) && c.level < OCT_MAXD;
+}
+
+// 3a. Children of each cell of one level: the number of non-empty digit
+// groups below `nch` (the dropped group makes no cell).
+__global__ void __launch_bounds__(256) k_oct_count(const OctCell* __restrict__ cells, int ls,
+                                                   int le, const uint64_t* __restrict__ k0,
+                                                   const uint64_t* __restrict__ k1,
+                                                   const uint64_t* __restrict__ k2, OctParams op,
+                                                   int* __restrict__ nchild) {
+  const int c = ls + blockIdx.x * 256 + threadIdx.x;
+  if (c >= le) return;
+  const OctCell cell = cells[c];
+  int cnt = 0;
+  if (oct_splits(cell, op)) {
+    const int nch = op.quad ? 4 : 8;
+    const int e = cell.start + cell.count;
+    int b = cell.start;
+    for (int d = 0; d < nch; ++d) {
+      int nb = oct_lower_bound(k0, k1, k2, b, e, cell.level, d + 1);
+      if (nb > b) ++cnt;
+      b = nb;
+    }
+  }
+  nchild[c - ls] = cnt;
+}
+
+// 3b. Write the children of one level at next + offset (offsets = exclusive
+// scan of the counts).
+__global__ void __launch_bounds__(256) k_oct_emit(OctCell* __restrict__ cells, int ls, int le,
+                                                  int next, const uint64_t* __restrict__ k0,
+                                                  const uint64_t* __restrict__ k1,
+                                                  const uint64_t* __restrict__ k2, OctParams op,
+                                                  const int* __restrict__ offs) {
+  const int c = ls + blockIdx.x * 256 + threadIdx.x;
+  if (c >= le) return;
+  OctCell cell = cells[c];
+  cell.child_first = -1;
+  cell.n_child = 0;
+  if (oct_splits(cell, op)) {
+    const int nch = op.quad ? 4 : 8;
+    const int e = cell.start + cell.count;
+    int b = cell.start;
+    int out = next + offs[c - ls];
+    cell.child_first = out;
+    for (int d = 0; d < nch; ++d) {
+      int nb = oct_lower_bound(k0, k1, k2, b, e, cell.level, d + 1);
+      if (nb > b) {
+        OctCell ch;
+        oct_child_box(cell.lo, cell.hi, d, op.quad, ch.lo, ch.hi);
+        ch.start = b;
+        ch.count = nb - b;
+        ch.level = cell.level + 1;
+        ch.parent = c;
+        ch.child_first = -1;
+        ch.n_child = 0;
+        cells[out++] = ch;
+        cell.n_child++;
+      }
+      b = nb;
+    }
+  }
+  cells[c].child_first = cell.child_first;
+  cells[c].n_child = cell.n_child;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// 4. Moments of every cell over its points (octree.c:237-266), one
+// wavefront per cell: lane sums over a stride of 64, then a fixed butterfly
+// (deterministic; the order differs from the reference's sequential sum by
+// rounding only).
+__global__ void __launch_bounds__(256) k_oct_moments(const OctCell* __restrict__ cells, int ncell,
+                                                     const LeafSeg* __restrict__ segs,
+                                                     double* __restrict__ mom) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ncell) return;
+  const OctCell cell = cells[c];
+  const int e = cell.start + cell.count;
+  double s[3] = {0, 0, 0};
+  for (int j = cell.start + lane; j < e; j += 64)
+    for (int a = 0; a < 3; ++a) s[a] += segs[j].a[a];
+  double com[3];
+  for (int a = 0; a < 3; ++a) com[a] = wave_sum(s[a]) * (1.0 / cell.count);
+  double g[3] = {0, 0, 0}, M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = cell.start + lane; j < e; j += 64) {
+    const LeafSeg q = segs[j];
+    double ds[3], sl[3];
+    for (int a = 0; a < 3; ++a) {
+      ds[a] = q.b[a] - q.a[a];
+      sl[a] = q.a[a] - com[a];
+      g[a] += ds[a];
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) M[3 * a + b] += ds[a] * sl[b];
+  }
+  for (int a = 0; a < 3; ++a) g[a] = wave_sum(g[a]);
+  for (int a = 0; a < 9; ++a) M[a] = wave_sum(M[a]);
+  if (lane == 0) {
+    double* o = mom + 15L * c;
+    for (int a = 0; a < 3; ++a) {
+      o[a] = com[a];
+      o[3 + a] = g[a];
+    }
+    for (int a = 0; a < 9; ++a) o[6 + a] = M[a];
+  }
+}
+
+// 5a. Subtree sizes, one level at a time from the deepest.
+__global__ void __launch_bounds__(256) k_oct_size(const OctCell* __restrict__ cells, int ls, int le,
+                                                  int* __restrict__ size) {
+  const int c = ls + blockIdx.x * 256 + threadIdx.x;
+  if (c >= le) return;
+  const OctCell cell = cells[c];
+  int sz = 1;
+  for (int k = 0; k < cell.n_child; ++k) sz += size[cell.child_first + k];
+  size[c] = sz;
+}
+
+// 5b. Preorder index of the children of one level's cells.
+__global__ void __launch_bounds__(256) k_oct_pre(const OctCell* __restrict__ cells, int ls, int le,
+                                                 const int* __restrict__ size,
+                                                 int* __restrict__ pre) {
+  const int c = ls + blockIdx.x * 256 + threadIdx.x;
+  if (c >= le) return;
+  const OctCell cell = cells[c];
+  int p = pre[c] + 1;
+  for (int k = 0; k < cell.n_child; ++k) {
+    pre[cell.child_first + k] = p;
+    p += size[cell.child_first + k];
+  }
+}
+
+// 5c. Traversal nodes in preorder.
+__global__ void __launch_bounds__(256) k_oct_nodes(const OctCell* __restrict__ cells, int ncell,
+                                                   const double* __restrict__ mom,
+                                                   const int* __restrict__ size,
+                                                   const int* __restrict__ pre,
+                                                   OctNode* __restrict__ nodes) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncell) return;
+  const OctCell cell = cells[c];
+  OctNode nd;
+  const double* m = mom + 15L * c;
+  for (int a = 0; a < 3; ++a) {
+    nd.com[a] = m[a];
+    nd.gam[a] = m[3 + a];
+  }
+  for (int a = 0; a < 9; ++a) nd.M[a] = m[6 + a];
+  nd.lmax = oct_max_len(cell.lo, cell.hi);
+  nd.lmin = oct_min_len(cell.lo, cell.hi);
+  nd.n = cell.count;
+  nd.first = cell.start;
+  nd.skip = size[c];
+  nd.pad = 0;
+  nodes[pre[c]] = nd;
+}
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+
+// One segment at r in FP64 with the reference's zero rules (tangle.c:486-514):
+// the estimate decides unless it lies within 1e-13 of a threshold, where the
+// reference's own expression does.
+__device__ inline void oct_segment(const double a[3], const double b[3], const double r[3],
+                                   double acc[3]) {
+  const double R[3] = {a[0] - r[0], a[1] - r[1], a[2] - r[2]};
+  const double Rp[3] = {b[0] - r[0], b[1] - r[1], b[2] - r[2]};
+  const double l2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+  const double lp2 = Rp[0] * Rp[0] + Rp[1] * Rp[1] + Rp[2] * Rp[2];
+  const double u = rsq64(l2), up = rsq64(lp2);
+  const double dab = R[0] * Rp[0] + R[1] * Rp[1] + R[2] * Rp[2];
+  const double w = u * up;
+  const double tt = fma(w, dab, 1.0);
+  const double nd1 = fabs(tt - 2.0);
+  bool zero = l2 < 1e-16 || lp2 < 1e-16 || nd1 < 1e-8;
+  const bool band = fabs(l2 - 1e-16) < 1e-22 || fabs(lp2 - 1e-16) < 1e-22 || fabs(nd1 - 1e-8) < 1e-13;
+  if (band) zero = guard_exact(a, b, r);
+  if (zero) return;
+  const double f = (u + up) * w * rcp64(tt);
+  acc[0] = fma(f, R[1] * Rp[2] - R[2] * Rp[1], acc[0]);
+  acc[1] = fma(f, R[2] * Rp[0] - R[0] * Rp[2], acc[1]);
+  acc[2] = fma(f, R[0] * Rp[1] - R[1] * Rp[0], acc[2]);
+}
+
+struct OctArgs {
+  const double* P;   // targets (SoA)
+  long cap;
+  int n;
+  int n_tgt_tiles;
+  const OctNode* nodes;
+  int n_nodes;
+  const LeafSeg* segs;
+  const ImgDev* img;
+  int n_img;
+  OctParams op;
+  double kappa4pi;
+  double* partial;   // [n_img][3][cap]
+  long* stats;       // [0] leaf pairs, [1] multipoles, [2] node visits (wave)
+};
+
+// Far-field value of a node at r (octree.c:296-330), kappa/4pi applied by the
+// caller.  Reference form: Gamma x R/|R|^3 + (M_21, 0, 0)/|R|^3 + (u x R)/|R|^5,
+// u_i = M_i2 R_2; dipole form: Gamma x R/|R|^3 - eps:M/|R|^3 + 3 (M R) x R/|R|^5.
+// R = r - centre, not minimum-imaged.
+__device__ inline void oct_multipole(const OctNode& nd, const double r[3], bool dipole,
+                                     double acc[3]) {
+  const double R[3] = {r[0] - nd.com[0], r[1] - nd.com[1], r[2] - nd.com[2]};
+  const double R2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+  const double ir = rsq64(R2);
+  const double i3 = ir * ir * ir;
+  const double i5 = i3 * ir * ir;
+  const double* M = nd.M;
+  double v[3] = {(nd.gam[1] * R[2] - nd.gam[2] * R[1]) * i3, (nd.gam[2] * R[0] - nd.gam[0] * R[2]) * i3,
+                 (nd.gam[0] * R[1] - nd.gam[1] * R[0]) * i3};
+  double u[3];
+  if (dipole) {
+    v[0] -= (M[5] - M[7]) * i3;
+    v[1] -= (M[6] - M[2]) * i3;
+    v[2] -= (M[1] - M[3]) * i3;
+    for (int i = 0; i < 3; ++i) u[i] = 3.0 * (M[3 * i] * R[0] + M[3 * i + 1] * R[1] + M[3 * i + 2] * R[2]);
+  } else {
+    v[0] += M[7] * i3;
+    for (int i = 0; i < 3; ++i) u[i] = M[3 * i + 2] * R[2];
+  }
+  v[0] += (u[1] * R[2] - u[2] * R[1]) * i5;
+  v[1] += (u[2] * R[0] - u[0] * R[2]) * i5;
+  v[2] += (u[0] * R[1] - u[1] * R[0]) * i5;
+  acc[0] += v[0];
+  acc[1] += v[1];
+  acc[2] += v[2];
+}
+
+// FP32 form of oct_multipole: R = r - centre in FP64, everything after in
+// FP32 (the node's moments rounded once).
+__device__ inline void oct_multipole32(const OctNode& nd, const double r[3], bool dipole,
+                                       double acc[3]) {
+  const float R[3] = {(float)(r[0] - nd.com[0]), (float)(r[1] - nd.com[1]), (float)(r[2] - nd.com[2])};
+  const float R2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+  const float ir = __builtin_amdgcn_rsqf(R2);
+  const float i3 = ir * ir * ir;
+  const float i5 = i3 * ir * ir;
+  float M[9], G[3];
+  for (int a = 0; a < 9; ++a) M[a] = (float)nd.M[a];
+  for (int a = 0; a < 3; ++a) G[a] = (float)nd.gam[a];
+  float v[3] = {(G[1] * R[2] - G[2] * R[1]) * i3, (G[2] * R[0] - G[0] * R[2]) * i3,
+                (G[0] * R[1] - G[1] * R[0]) * i3};
+  float u[3];
+  if (dipole) {
+    v[0] -= (M[5] - M[7]) * i3;
+    v[1] -= (M[6] - M[2]) * i3;
+    v[2] -= (M[1] - M[3]) * i3;
+    for (int i = 0; i < 3; ++i) u[i] = 3.f * (M[3 * i] * R[0] + M[3 * i + 1] * R[1] + M[3 * i + 2] * R[2]);
+  } else {
+    v[0] += M[7] * i3;
+    for (int i = 0; i < 3; ++i) u[i] = M[3 * i + 2] * R[2];
+  }
+  v[0] += (u[1] * R[2] - u[2] * R[1]) * i5;
+  v[1] += (u[2] * R[0] - u[0] * R[2]) * i5;
+  v[2] += (u[0] * R[1] - u[1] * R[0]) * i5;
+  acc[0] += (double)v[0];
+  acc[1] += (double)v[1];
+  acc[2] += (double)v[2];
+}
+
+// Grid: blockIdx.x = group of 4 target tiles, blockIdx.y = image.
+// MIXED: multipoles in FP32 (leaf pairs and all decisions in FP64).
+template <bool MIXED>
+__device__ __forceinline__ void octree_body(const OctArgs& A) {
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int T = blockIdx.x * 4 + wid;
+  if (T >= A.n_tgt_tiles) return;
+  const int I = blockIdx.y;
+  const ImgDev im = A.img[I];
+  const bool primary = im.primary != 0;
+  const int s = T * TILE + lane;
+  const bool valid = s < A.n;
+  const int st = valid ? s : T * TILE;
+  double r[3] = {A.P[st], A.P[A.cap + st], A.P[2 * A.cap + st]};
+  double rs[3] = {r[0] - im.shift[0], r[1] - im.shift[1], r[2] - im.shift[2]};
+  if (im.axis >= 0) rs[im.axis] = im.wall2 - r[im.axis];
+  const int skip_id = (primary && valid) ? s : -1;
+  double acc[3] = {0, 0, 0};
+  long leaf_pairs = 0, mps = 0, visits = 0;
+  int next = 0;   // this lane resumes at node >= next
+  int node = 0;
+  while (node < A.n_nodes) {
+    const OctNode nd = A.nodes[node];
+    ++visits;
+    bool open = false;
+    if (node >= next) {
+      if (nd.n == 0) {
+        next = node + nd.skip;
+      } else if (nd.n == 1 || nd.lmin < A.op.grain) {
+        for (int j = nd.first; j < nd.first + nd.n; ++j) {
+          const LeafSeg q = A.segs[j];
+          if (q.m == skip_id || q.fwd == skip_id) continue;
+          oct_segment(q.a, q.b, rs, acc);
+          ++leaf_pairs;
+        }
+        next = node + nd.skip;
+      } else {
+        double cw[3];
+        pwrap(rs, nd.com, A.op.box, cw);
+        const double dv[3] = {rs[0] - cw[0], rs[1] - cw[1], rs[2] - cw[2]};
+        const double d = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+        if (d > 0 && nd.lmax / d < A.op.theta) {
+          if (MIXED)
+            oct_multipole32(nd, rs, A.op.dipole != 0, acc);
+          else
+            oct_multipole(nd, rs, A.op.dipole != 0, acc);
+          ++mps;
+          next = node + nd.skip;
+        } else {
+          open = true;
+          next = node + 1;
+        }
+      }
+    }
+    node = __builtin_amdgcn_ballot_w64(open) != 0 ? node + 1 : node + nd.skip;
+  }
+  if (im.axis >= 0) acc[im.axis] = -acc[im.axis];
+  if (valid) {
+    double* out = A.partial + (long)I * 3 * A.cap;
+    out[s] = acc[0] * A.kappa4pi;
+    out[A.cap + s] = acc[1] * A.kappa4pi;
+    out[2 * A.cap + s] = acc[2] * A.kappa4pi;
+    if (A.stats) {
+      atomicAdd((unsigned long long*)&A.stats[0], (unsigned long long)leaf_pairs);
+      atomicAdd((unsigned long long*)&A.stats[1], (unsigned long long)mps);
+    }
+  }
+  if (A.stats && lane == 0) atomicAdd((unsigned long long*)&A.stats[2], (unsigned long long)visits);
+}
+
+__global__ void __launch_bounds__(256) k_octree_fp64(OctArgs A) { octree_body<false>(A); }
+__global__ void __launch_bounds__(256) k_octree_mixed(OctArgs A) { octree_body<true>(A); }
+
+}  // namespace ov

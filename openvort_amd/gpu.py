@@ -31,6 +31,7 @@ SYMBOLS = (
     "ov_ctx_reset_stats",
     "ov_ctx_last_interactions",
     "ov_ctx_last_device_ms",
+    "ov_ctx_tree_stats",
 )
 
 _lib = None
@@ -78,6 +79,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.ov_ctx_last_interactions.restype = C.c_double
     lib.ov_ctx_last_device_ms.argtypes = [C.c_void_p]
     lib.ov_ctx_last_device_ms.restype = C.c_double
+    lib.ov_ctx_tree_stats.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double)]
+    lib.ov_ctx_tree_stats.restype = None
     _lib = lib
     return lib
 
@@ -147,6 +150,12 @@ class Context:
 
     def last_device_ms(self) -> float:
         return float(self.lib.ov_ctx_last_device_ms(self.ptr))
+
+    def tree_stats(self):
+        """(cells, leaf pairs, multipoles) of the octree evaluations since the last call."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self.lib.ov_ctx_tree_stats(self.ptr, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
 
     def stream(self) -> int:
         return int(self.lib.ov_ctx_stream(self.ptr) or 0)

@@ -332,7 +332,7 @@ static void ot_moments(otnode *c, const ov_tangle_state *t) {
  * boxes; each point goes to the first child, in child order, that contains
  * it strictly; points on shared faces or outside every child are dropped. */
 static void ot_split(otnode *c, const ov_tangle_state *t, const ov_params *p) {
-  ot_cells++;
+  if (c->n > 0) ot_cells++;
   ot_moments(c, t);
   if (c->n <= 1) return;
   if (box_min_len(c->lo, c->hi) < p->BH_grain * p->dl_max) return;
@@ -380,8 +380,9 @@ static otnode *ot_build(const ov_tangle_state *t, const ov_params *p) {
 
 /* Sum over the listed segments at r, skipping segments that start or end at
  * `skip` (tangle.c:553-576). */
-static void sum_segments(const ov_tangle_state *t, const int *ids, int n, const double r[3],
+static long sum_segments(const ov_tangle_state *t, const int *ids, int n, const double r[3],
                          int skip, double kappa, double out[3]) {
+  long cnt = 0;
   out[0] = out[1] = out[2] = 0;
   for (int q = 0; q < n; ++q) {
     int m = ids ? ids[q] : q;
@@ -391,7 +392,9 @@ static void sum_segments(const ov_tangle_state *t, const int *ids, int n, const 
     segment_of(t, m, a, b);
     ovo_segment_field(a, b, r, kappa, v);
     for (int i = 0; i < 3; ++i) out[i] += v[i];
+    ++cnt;
   }
+  return cnt;
 }
 
 /* Far-field value of a cell at r.  Reference form (octree.c:296-330):
@@ -437,8 +440,7 @@ static void ot_vs(const otnode *c, const ov_tangle_state *t, const double r[3], 
   double Lm = box_max_len(c->lo, c->hi), Lmin = box_min_len(c->lo, c->hi);
   double d = chord(r, c->com, &t->box);
   if (c->n == 1 || Lmin < p->BH_grain) {
-    sum_segments(t, c->ids, c->n, r, skip, p->kappa, out);
-    *pairs += c->n;
+    *pairs += sum_segments(t, c->ids, c->n, r, skip, p->kappa, out);
     return;
   }
   if (d > 0 && Lm / d < p->BH_resolution) {

@@ -42,7 +42,8 @@ void ovo_enforce_boundaries(ov_tangle_state *t);
 void ovo_segment_field(const double a[3], const double b[3], const double r[3],
                        double kappa, double out[3]);
 
-/* Octree statistics of the last ovo_update_velocities with use_BH. */
+/* Octree statistics of the last ovo_update_velocities with use_BH:
+ * non-empty cells, evaluated leaf segment-point pairs, multipoles. */
 void ovo_octree_stats(long *n_cells, long *n_leaf_pairs, long *n_multipoles);
 
 /* Threads used by the OpenMP loops (0: runtime default). */
