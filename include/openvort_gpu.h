@@ -159,6 +159,20 @@ int ov_gpu_rk4_step(ov_ctx *ctx, ov_tangle_state *tangle, double t, double dt,
 int ov_gpu_rk4_run(ov_ctx *ctx, ov_tangle_state *tangle, double t0, double dt,
                    int nsteps, const ov_params *params, int flags);
 
+/* Ensembles: nsteps RK4 steps (update_tangle + rk4_step each, as
+ * ov_gpu_rk4_run) of ntangles independent tangles, params[k] for tangle k
+ * (each with its own box, images, flows and friction; all with one precision
+ * and RK variant; direct sum only).  Every velocity evaluation of the whole
+ * ensemble is one persistent launch over a queue of work units; one RK4 step
+ * of the ensemble is captured once as a HIP graph and replayed; unit size
+ * and grid are picked by timing candidates on the first call for a layout.
+ * No counterpart in the reference (it holds one tangle per process). */
+int ov_gpu_rk4_run_batch(ov_ctx *ctx, ov_tangle_state *const *tangles, int ntangles,
+                         double t0, double dt, int nsteps, const ov_params *params, int flags);
+/* Unit size (work items per unit), persistent grid (blocks of 4 waves) and
+ * the tuning time of the last batch layout. */
+void ov_ctx_batch_tuning(const ov_ctx *ctx, int *unit, int *blocks, double *tune_ms);
+
 /* Per-kernel timings gathered while OV_PROFILE was set; kernel names are the
  * device function names.  Returns the number of launches, 0 if none. */
 long ov_ctx_kernel_stats(ov_ctx *ctx, const char *kernel, double *total_ms);

@@ -20,6 +20,7 @@
 #include "ov_device.h"
 #include "ov_kernels.hip"
 #include "ov_octree.hip"
+#include "ov_batch.hip"
 
 #include <hipcub/hipcub.hpp>
 
@@ -105,6 +106,22 @@ struct ov_ctx {
   double tree_leaf_pairs = 0, tree_multipoles = 0;
   hipEvent_t t_begin = nullptr, t_end = nullptr;
   double last_device_ms = 0;
+
+  // ensembles
+  int b_K = 0, b_ntot = 0, b_rec_tot = 0, b_n_tgt = 0, b_mixed = 0;
+  std::vector<TangleDev> b_tg;
+  std::vector<std::vector<int>> b_act;
+  TangleDev* b_dtg = nullptr;
+  int *b_tid = nullptr, *b_tile_tangle = nullptr, *b_tgt_tangle = nullptr, *b_tM = nullptr,
+      *b_tS = nullptr, *b_ucnt = nullptr, *b_urank = nullptr, *b_ubsum = nullptr,
+      *b_ucounts = nullptr, *b_queue = nullptr;
+  ImgDev* b_img = nullptr;
+  double* b_upart = nullptr;
+  size_t b_upart_bytes = 0;
+  long b_cap_tid = 0, b_cap_rec = 0, b_cap_tgt = 0, b_cap_img = 0, b_cap_K = 0;
+  std::map<std::pair<long, int>, std::pair<int, int>> b_tuned;   // (ntot, K) -> (U, blocks)
+  int b_U = 256, b_blocks = 1024;
+  double b_tune_ms = 0;
 };
 
 namespace {
@@ -405,6 +422,33 @@ int download_scalar(ov_ctx* ctx, const double* src, double* dst) {
   if (rc) return rc;
   double* h = ctx->hst;
   HIPCHK(hipMemcpyAsync(h, src, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < n; ++q) dst[ctx->act[q]] = h[q];
+  return OV_OK;
+}
+
+// As download_vec3 / download_scalar for a tangle placed at offset o of the planes.
+int download_vec3_at(ov_ctx* ctx, const double* src, long o, ov_vec3d* dst) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  double* h = ctx->hst;
+  for (int i = 0; i < 3; ++i)
+    HIPCHK(hipMemcpyAsync(h + i * (long)n, src + i * cap + o, sizeof(double) * n, hipMemcpyDeviceToHost,
+                          ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < n; ++q) {
+    double* v = dst[ctx->act[q]].p;
+    v[0] = h[q];
+    v[1] = h[n + q];
+    v[2] = h[2 * (long)n + q];
+  }
+  return OV_OK;
+}
+
+int download_scalar_at(ov_ctx* ctx, const double* src, long o, double* dst) {
+  const int n = ctx->n;
+  double* h = ctx->hst;
+  HIPCHK(hipMemcpyAsync(h, src + o, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   for (int q = 0; q < n; ++q) dst[ctx->act[q]] = h[q];
   return OV_OK;
@@ -771,6 +815,128 @@ int finish(ov_ctx* ctx, int* flag_out) {
   return OV_OK;
 }
 
+// ---- ensembles ----------------------------------------------------------------
+
+int grow_int(ov_ctx* ctx, int** p, long* cap, long need) {
+  if (need <= *cap) return OV_OK;
+  long c = std::max<long>(need, 2 * *cap);
+  int rc = dalloc(ctx, (void**)p, sizeof(int) * c);
+  if (!rc) *cap = c;
+  return rc;
+}
+
+// One velocity evaluation of every tangle of the batch at positions X.
+int evaluate_batch(ov_ctx* ctx, const double* X, double* Tg, double* Ng, double* dxi, double* vs,
+                   double* vel, float* direct_ms) {
+  const long cap = ctx->cap;
+  const int ntot = ctx->b_ntot, rec_tot = ctx->b_rec_tot, n_tgt = ctx->b_n_tgt;
+  const int nb = (ntot + 255) / 256, nbu = (n_tgt + 255) / 256;
+  int rc;
+  if ((rc = launch(ctx, "kb_geometry", kb_geometry, dim3(nb), dim3(256), X, (const int*)ctx->fwd,
+                   (const int*)ctx->rev, cap, (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, 1, Tg,
+                   Ng, dxi, ctx->lia, ntot)))
+    return rc;
+  if ((rc = launch(ctx, "kb_cont", kb_cont, dim3(nb), dim3(256), X, (const int*)ctx->fwd, cap,
+                   (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, ctx->brk, ctx->rank, ctx->bsum,
+                   ntot)))
+    return rc;
+  if ((rc = launch(ctx, "k_scan_blocks", k_scan_blocks, dim3(1), dim3(1024), ctx->bsum, nb, 0, ctx->counts)))
+    return rc;
+  if ((rc = launch(ctx, "kb_records", kb_records, dim3(nb), dim3(256), X, (const int*)ctx->fwd, cap,
+                   (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, (const int*)ctx->brk,
+                   (const int*)ctx->rank, (const int*)ctx->bsum, ctx->near, ctx->b_tM, ctx->b_tS, ntot)))
+    return rc;
+  if ((rc = launch(ctx, "kb_src_tiles", kb_src_tiles, dim3(rec_tot / TILE), dim3(TILE), ctx->near,
+                   (const int*)ctx->b_tile_tangle, (const TangleDev*)ctx->b_dtg, (const int*)ctx->b_tM,
+                   (const int*)ctx->b_tS, ctx->src, ctx->far)))
+    return rc;
+  if ((rc = launch(ctx, "kb_tgt_tiles", kb_tgt_tiles, dim3(n_tgt), dim3(TILE), X, cap,
+                   (const int*)ctx->b_tgt_tangle, (const TangleDev*)ctx->b_dtg, ctx->tgt)))
+    return rc;
+  if ((rc = launch(ctx, "kb_adjacent", kb_adjacent, dim3((unsigned)((rec_tot + 255) / 256)), dim3(256),
+                   (const NearRec*)ctx->near, (const int*)ctx->b_tile_tangle, (const TangleDev*)ctx->b_dtg,
+                   (const int*)ctx->b_tM, ctx->adj, rec_tot)))
+    return rc;
+  if ((rc = launch(ctx, "kb_units", kb_units, dim3(nbu), dim3(256), (const int*)ctx->b_tgt_tangle,
+                   (const TangleDev*)ctx->b_dtg, (const int*)ctx->b_tS, ctx->b_U, ctx->b_ucnt, ctx->b_urank,
+                   ctx->b_ubsum, n_tgt)))
+    return rc;
+  if ((rc = launch(ctx, "k_scan_blocks", k_scan_blocks, dim3(1), dim3(1024), ctx->b_ubsum, nbu, 0,
+                   ctx->b_ucounts)))
+    return rc;
+  if ((rc = launch(ctx, "kb_reset", kb_reset, dim3(1), dim3(64), ctx->b_queue))) return rc;
+  BatchArgs A;
+  A.P = X;
+  A.cap = cap;
+  A.tg = ctx->b_dtg;
+  A.tgt_tangle = ctx->b_tgt_tangle;
+  A.n_tgt = n_tgt;
+  A.tgt = ctx->tgt;
+  A.src = ctx->src;
+  A.far = ctx->far;
+  A.near = ctx->near;
+  A.img = ctx->b_img;
+  A.adj = ctx->adj;
+  A.tS = ctx->b_tS;
+  A.urank = ctx->b_urank;
+  A.ubsum = ctx->b_ubsum;
+  A.ucounts = ctx->b_ucounts;
+  A.queue = ctx->b_queue;
+  A.U = ctx->b_U;
+  A.rnear2 = ctx->rnear * ctx->rnear;
+  A.upart = ctx->b_upart;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (direct_ms) {
+    e0 = get_event(ctx);
+    e1 = get_event(ctx);
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+  }
+  if (ctx->b_mixed)
+    rc = launch(ctx, "kb_direct_mixed", kb_direct_mixed, dim3(ctx->b_blocks), dim3(256), A);
+  else
+    rc = launch(ctx, "kb_direct_fp64", kb_direct_fp64, dim3(ctx->b_blocks), dim3(256), A);
+  if (rc) return rc;
+  if (direct_ms) {
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(direct_ms, e0, e1));
+    ctx->ev_pool.push_back(e0);
+    ctx->ev_pool.push_back(e1);
+  }
+  if ((rc = launch(ctx, "kb_reduce", kb_reduce, dim3(nb), dim3(256), (const double*)ctx->b_upart,
+                   (const int*)ctx->b_ucnt, (const int*)ctx->b_urank, (const int*)ctx->b_ubsum,
+                   (const int*)ctx->b_tid, cap, ctx->bs, ntot)))
+    return rc;
+  return launch(ctx, "kb_epilogue", kb_epilogue, dim3(nb), dim3(256), (const double*)Tg, (const double*)Ng,
+                (const double*)ctx->lia, (const double*)ctx->bs, cap, (const int*)ctx->b_tid,
+                (const TangleDev*)ctx->b_dtg, vs, vel, ctx->flag, ntot);
+}
+
+// update_tangle + rk4_step of every tangle of the batch (X0 device-resident).
+int rk4_batch(ov_ctx* ctx, double dt, int rk_variant, bool enforce) {
+  const long cap = ctx->cap;
+  const int ntot = ctx->b_ntot;
+  int rc;
+  if ((rc = evaluate_batch(ctx, ctx->X0, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0], nullptr)))
+    return rc;
+  const double h[3] = {dt / 2, dt / 2, rk_variant == OV_RK_CLASSIC ? dt : dt / 2};
+  for (int st = 0; st < 3; ++st) {
+    if ((rc = launch(ctx, "k_stage", k_stage, grid1(ntot), dim3(256), (const double*)ctx->X0,
+                     (const double*)ctx->K[st], h[st], ntot, cap, ctx->Xs)))
+      return rc;
+    if ((rc = evaluate_batch(ctx, ctx->Xs, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[st + 1], nullptr)))
+      return rc;
+  }
+  if ((rc = launch(ctx, "k_combine", k_combine, grid1(ntot), dim3(256), ctx->X0, (const double*)ctx->K[0],
+                   (const double*)ctx->K[1], (const double*)ctx->K[2], (const double*)ctx->K[3], dt, ntot,
+                   cap, ctx->flag)))
+    return rc;
+  if (enforce)
+    return launch(ctx, "kb_wrap", kb_wrap, grid1(ntot), dim3(256), ctx->X0, cap, (const int*)ctx->b_tid,
+                  (const TangleDev*)ctx->b_dtg, ntot);
+  return OV_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -833,6 +999,11 @@ void ov_ctx_destroy(ov_ctx* ctx) {
                 ctx->t_maxd, ctx->t_segs, ctx->t_cells, ctx->t_nchild, ctx->t_offs, ctx->t_size,
                 ctx->t_pre, ctx->t_mom, ctx->t_nodes, ctx->t_cub, ctx->t_stats};
   for (void* b : tb)
+    if (b) hipFree(b);
+  void* bb[] = {ctx->b_dtg,  ctx->b_tid,   ctx->b_tile_tangle, ctx->b_tgt_tangle, ctx->b_tM,
+                ctx->b_tS,   ctx->b_ucnt,  ctx->b_urank,       ctx->b_ubsum,      ctx->b_ucounts,
+                ctx->b_queue, ctx->b_img,  ctx->b_upart};
+  for (void* b : bb)
     if (b) hipFree(b);
   if (ctx->hst) hipHostFree(ctx->hst);
   for (auto& p : ctx->pending) {
@@ -969,6 +1140,239 @@ static int rk4_common(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int
   }
   if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite %s", (flag & 2) ? "position" : "velocity");
   return OV_OK;
+}
+
+int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntangles, double t0,
+                         double dt, int nsteps, const ov_params* params, int flags) {
+  if (!ctx) return OV_E_INVALID;
+  if (!tangles || !params || ntangles <= 0 || nsteps < 0)
+    return fail(ctx, OV_E_INVALID, "batch: bad arguments");
+  HIPCHK(hipSetDevice(ctx->device));
+  (void)t0;
+  const int K = ntangles;
+  const int prec = params[0].precision, rkv = params[0].rk_variant;
+  ctx->b_tg.assign(K, TangleDev{});
+  ctx->b_act.assign(K, {});
+  std::vector<std::vector<int>> fwd_l(K), rev_l(K);
+  std::vector<ImgDev> imgs;
+  std::vector<int> npts(K);
+  double rnear = 0;
+  for (int k = 0; k < K; ++k) {
+    if (params[k].use_BH) return fail(ctx, OV_E_UNSUPPORTED, "batch: octree not supported (tangle %d)", k);
+    if (params[k].precision != prec || params[k].rk_variant != rkv)
+      return fail(ctx, OV_E_INVALID, "batch: tangles must share precision and RK variant");
+    int rc = prepare(ctx, tangles[k], &params[k]);
+    if (rc) {
+      char msg[512];
+      snprintf(msg, sizeof msg, "tangle %d: %s", k, ctx->err);
+      return fail(ctx, rc, "%s", msg);
+    }
+    if (ctx->n == 0) return fail(ctx, OV_E_INVALID, "batch: tangle %d is empty", k);
+    TangleDev& t = ctx->b_tg[k];
+    t.box = ctx->box;
+    t.ph = ctx->ph;
+    t.kappa4pi = ctx->ph.kappa / 4 / 3.14159265358979323846;
+    t.n = ctx->n;
+    t.lia_only = ctx->lia_only;
+    t.wall_mask = ctx->wall_periodic_mask;
+    t.img_base = (int)imgs.size();
+    t.n_img = (int)ctx->imgs.size();
+    imgs.insert(imgs.end(), ctx->imgs.begin(), ctx->imgs.end());
+    ctx->b_act[k] = ctx->act;
+    fwd_l[k] = ctx->fwd_s;
+    rev_l[k] = ctx->rev_s;
+    npts[k] = ctx->n;
+    rnear = ctx->rnear;
+  }
+  ctx->rnear = rnear;
+  ctx->b_mixed = prec == OV_PREC_MIXED;
+  // layout
+  long pt = 0, rec = 0;
+  for (int k = 0; k < K; ++k) {
+    TangleDev& t = ctx->b_tg[k];
+    t.pt_base = (int)pt;
+    t.tgt_base = (int)(pt / TILE);
+    t.n_tgt = (t.n + TILE - 1) / TILE;
+    pt += (long)t.n_tgt * TILE;
+    t.rec_base = (int)rec;
+    rec += (long)((2 * t.n + TILE - 1) / TILE) * TILE;
+  }
+  const int ntot = (int)pt, rec_tot = (int)rec, n_tgt = ntot / TILE;
+  ctx->b_K = K;
+  ctx->b_ntot = ntot;
+  ctx->b_rec_tot = rec_tot;
+  ctx->b_n_tgt = n_tgt;
+  ctx->n = ntot;
+  int rc = ensure_capacity(ctx, std::max<long>(ntot, (rec_tot + 1) / 2));
+  if (rc) return rc;
+  ctx->cap = ctx->buf_cap;
+  const long cap = ctx->cap;
+  std::vector<int> tid(ntot, -1), fwd(ntot), rev(ntot), tile_tangle(rec_tot / TILE), tgt_tangle(n_tgt);
+  for (int s = 0; s < ntot; ++s) fwd[s] = rev[s] = s;
+  for (int k = 0; k < K; ++k) {
+    const TangleDev& t = ctx->b_tg[k];
+    for (int q = 0; q < t.n; ++q) {
+      tid[t.pt_base + q] = k;
+      fwd[t.pt_base + q] = t.pt_base + fwd_l[k][q];
+      rev[t.pt_base + q] = t.pt_base + rev_l[k][q];
+    }
+    for (int j = t.rec_base / TILE; j < (t.rec_base + ((2 * t.n + TILE - 1) / TILE) * TILE) / TILE; ++j)
+      tile_tangle[j] = k;
+    for (int T = t.tgt_base; T < t.tgt_base + t.n_tgt; ++T) tgt_tangle[T] = k;
+  }
+  if ((rc = grow_int(ctx, &ctx->b_tid, &ctx->b_cap_tid, ntot)) ||
+      (rc = grow_int(ctx, &ctx->b_tile_tangle, &ctx->b_cap_rec, rec_tot / TILE + 1)))
+    return rc;
+  if (n_tgt > ctx->b_cap_tgt) {
+    long c = std::max<long>(n_tgt, 2 * ctx->b_cap_tgt);
+    if ((rc = dalloc(ctx, (void**)&ctx->b_tgt_tangle, sizeof(int) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->b_ucnt, sizeof(int) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->b_urank, sizeof(int) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->b_ubsum, sizeof(int) * (c / 256 + 2))))
+      return rc;
+    ctx->b_cap_tgt = c;
+  }
+  if (K > ctx->b_cap_K) {
+    long c = std::max<long>(K, 2 * ctx->b_cap_K);
+    if ((rc = dalloc(ctx, (void**)&ctx->b_dtg, sizeof(TangleDev) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->b_tM, sizeof(int) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->b_tS, sizeof(int) * c)))
+      return rc;
+    ctx->b_cap_K = c;
+  }
+  if ((long)imgs.size() > ctx->b_cap_img) {
+    long c = std::max<long>((long)imgs.size(), 2 * ctx->b_cap_img);
+    if ((rc = dalloc(ctx, (void**)&ctx->b_img, sizeof(ImgDev) * c))) return rc;
+    ctx->b_cap_img = c;
+  }
+  if (!ctx->b_ucounts && (rc = dalloc(ctx, (void**)&ctx->b_ucounts, 256))) return rc;
+  if (!ctx->b_queue && (rc = dalloc(ctx, (void**)&ctx->b_queue, 256))) return rc;
+  // unit partials for the smallest unit size the tuner tries
+  const int Us[4] = {128, 256, 512, 1024};
+  size_t units = 0;
+  for (int k = 0; k < K; ++k) {
+    const TangleDev& t = ctx->b_tg[k];
+    const long items = (long)((2 * t.n + TILE - 1) / TILE) * t.n_img;
+    units += (size_t)t.n_tgt * (size_t)((items + Us[0] - 1) / Us[0]);
+  }
+  const size_t ub = sizeof(double) * 3 * TILE * std::max<size_t>(units, 1);
+  if (ub > ctx->b_upart_bytes) {
+    if ((rc = dalloc(ctx, (void**)&ctx->b_upart, ub))) return rc;
+    ctx->b_upart_bytes = ub;
+  }
+  // upload
+  HIPCHK(hipMemcpyAsync(ctx->b_tid, tid.data(), sizeof(int) * ntot, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->b_tile_tangle, tile_tangle.data(), sizeof(int) * tile_tangle.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->b_tgt_tangle, tgt_tangle.data(), sizeof(int) * n_tgt, hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->fwd, fwd.data(), sizeof(int) * ntot, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->rev, rev.data(), sizeof(int) * ntot, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->b_dtg, ctx->b_tg.data(), sizeof(TangleDev) * K, hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->b_img, imgs.data(), sizeof(ImgDev) * imgs.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  for (int q = 0; q < 4; ++q) HIPCHK(hipMemsetAsync(ctx->K[q], 0, sizeof(double) * 3 * cap, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->flag, 0, sizeof(int), ctx->stream));
+  if ((rc = ensure_host(ctx, sizeof(double) * 3 * (size_t)cap))) return rc;
+  {
+    double* h = ctx->hst;
+    for (int k = 0; k < K; ++k) {
+      const TangleDev& t = ctx->b_tg[k];
+      const ov_vec3d* src = tangles[k]->vnodes;
+      for (int q = 0; q < t.n_tgt * TILE; ++q) {
+        const int s = t.pt_base + q;
+        const double* v = src[ctx->b_act[k][q < t.n ? q : 0]].p;   // padding: a copy of a point
+        h[s] = v[0];
+        h[cap + s] = v[1];
+        h[2 * cap + s] = v[2];
+      }
+    }
+    HIPCHK(hipMemcpyAsync(ctx->X0, h, sizeof(double) * 3 * cap, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->profile = false;
+  // tuner: unit size x persistent grid, timed on one evaluation of these tangles
+  const auto key = std::make_pair((long)ntot, K);
+  auto it = ctx->b_tuned.find(key);
+  if (it == ctx->b_tuned.end()) {
+    int dev_cus = 256;
+    hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    float best = 1e30f;
+    std::pair<int, int> pick(256, dev_cus * 4);
+    double tune_ms = 0;
+    for (int U : Us) {
+      for (int m : {2, 4, 8}) {
+        ctx->b_U = U;
+        ctx->b_blocks = dev_cus * m;
+        float ms = 0;
+        if ((rc = evaluate_batch(ctx, ctx->X0, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0], &ms))) return rc;
+        tune_ms += ms;
+        if (ms < best) {
+          best = ms;
+          pick = {U, dev_cus * m};
+        }
+      }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->b_tune_ms = tune_ms;
+    it = ctx->b_tuned.emplace(key, pick).first;
+    for (int q = 0; q < 4; ++q) HIPCHK(hipMemsetAsync(ctx->K[q], 0, sizeof(double) * 3 * cap, ctx->stream));
+  }
+  ctx->b_U = it->second.first;
+  ctx->b_blocks = it->second.second;
+  // one RK4 step captured as a graph, replayed nsteps times
+  const bool enforce = (flags & OV_ENFORCE_BOUNDARIES) != 0;
+  if ((rc = mark_begin(ctx))) return rc;
+  if (nsteps > 0) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = rk4_batch(ctx, dt, rkv, enforce);
+    hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return fail(ctx, OV_E_DEVICE, "graph capture: %s", hipGetErrorString(ec));
+    ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ec != hipSuccess) {
+      hipGraphDestroy(graph);
+      return fail(ctx, OV_E_DEVICE, "graph instantiate: %s", hipGetErrorString(ec));
+    }
+    for (int i = 0; i < nsteps && ec == hipSuccess; ++i) ec = hipGraphLaunch(exec, ctx->stream);
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    if (ec != hipSuccess) return fail(ctx, OV_E_DEVICE, "graph launch: %s", hipGetErrorString(ec));
+  }
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  // results back into each tangle
+  for (int k = 0; k < K; ++k) {
+    const TangleDev& t = ctx->b_tg[k];
+    ctx->act = ctx->b_act[k];
+    ctx->n = t.n;
+    const long o = t.pt_base;
+    if ((rc = download_vec3_at(ctx, ctx->X0, o, tangles[k]->vnodes))) return rc;
+    if (nsteps > 0) {
+      if ((rc = download_vec3_at(ctx, ctx->K[0], o, tangles[k]->vels)) ||
+          (rc = download_vec3_at(ctx, ctx->vs1, o, tangles[k]->vs)) ||
+          (rc = download_vec3_at(ctx, ctx->T1, o, tangles[k]->tangents)) ||
+          (rc = download_vec3_at(ctx, ctx->N1, o, tangles[k]->normals)) ||
+          (rc = download_scalar_at(ctx, ctx->dxi1, o, tangles[k]->dxi)))
+        return rc;
+    }
+  }
+  ctx->n = ntot;
+  if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite %s", (flag & 2) ? "position" : "velocity");
+  return OV_OK;
+}
+
+void ov_ctx_batch_tuning(const ov_ctx* ctx, int* unit, int* blocks, double* tune_ms) {
+  if (!ctx) return;
+  if (unit) *unit = ctx->b_U;
+  if (blocks) *blocks = ctx->b_blocks;
+  if (tune_ms) *tune_ms = ctx->b_tune_ms;
 }
 
 int ov_gpu_rk4_step(ov_ctx* ctx, ov_tangle_state* t, double time, double dt, const ov_params* p,

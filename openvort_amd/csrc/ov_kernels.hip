@@ -87,15 +87,13 @@ __device__ __forceinline__ double rcp64(double x) {
 // applied to the minimum-image differences s_j - s_0.  LIA (tangle.c:530-551):
 //   v_LIA = kappa/4pi ln(2 sqrt(l+ l-) / (e^(1/4) a0)) s' x s''.
 // With compute_geom = 0 the caller's s', s'' (T, Nn) are kept.
-__global__ void __launch_bounds__(256) k_geometry(const double* __restrict__ P,
-                                                  const int* __restrict__ fwd,
-                                                  const int* __restrict__ rev, int n, long cap,
-                                                  BoxDev box, PhysDev ph, int compute_geom,
-                                                  double* __restrict__ T, double* __restrict__ Nn,
-                                                  double* __restrict__ dxi,
-                                                  double* __restrict__ lia) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+__device__ __forceinline__ void geometry_point(int s, const double* __restrict__ P,
+                                               const int* __restrict__ fwd,
+                                               const int* __restrict__ rev, long cap,
+                                               const BoxDev& box, const PhysDev& ph,
+                                               int compute_geom, double* __restrict__ T,
+                                               double* __restrict__ Nn, double* __restrict__ dxi,
+                                               double* __restrict__ lia) {
   int f = fwd[s], r = rev[s];
   double p0[3], pf[3], pr[3];
   load3(P, cap, s, p0);
@@ -146,6 +144,18 @@ __global__ void __launch_bounds__(256) k_geometry(const double* __restrict__ P,
   double v[3] = {(t[1] * nm[2] - t[2] * nm[1]) * beta, (t[2] * nm[0] - t[0] * nm[2]) * beta,
                  (t[0] * nm[1] - t[1] * nm[0]) * beta};
   store3(lia, cap, s, v);
+}
+
+__global__ void __launch_bounds__(256) k_geometry(const double* __restrict__ P,
+                                                  const int* __restrict__ fwd,
+                                                  const int* __restrict__ rev, int n, long cap,
+                                                  BoxDev box, PhysDev ph, int compute_geom,
+                                                  double* __restrict__ T, double* __restrict__ Nn,
+                                                  double* __restrict__ dxi,
+                                                  double* __restrict__ lia) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  geometry_point(s, P, fwd, rev, cap, box, ph, compute_geom, T, Nn, dxi, lia);
 }
 
 // ---------------------------------------------------------------------------
@@ -665,48 +675,42 @@ __device__ __forceinline__ void far_tile(const FarRec* __restrict__ rec,
   }
 }
 
-// Grid: blockIdx.x = group of 4 target tiles (one wavefront each),
-// blockIdx.y = chunk of the (source tile, image) work list.  Each wave writes
-// its chunk's sum to partial[chunk]; k_reduce adds the chunks in order.
-// MIXED: FP32 pass for tiles beyond the near radius; otherwise all FP64.
-template <bool MIXED>
-__device__ __forceinline__ void direct_body(const DirectArgs& A) {
-  const int lane = threadIdx.x & 63;
-  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int T = blockIdx.x * 4 + wid;
-  if (T >= A.n_tgt_tiles) return;
-  const int chunk = blockIdx.y;
-  const long n_work = (long)A.counts[1] * A.n_img;
-  const long w0 = (n_work * chunk) / A.n_chunks;
-  const long w1 = (n_work * (chunk + 1)) / A.n_chunks;
-  const int s = T * TILE + lane;
-  const bool valid = s < A.n;
-  const TgtTile tt = A.tgt[T];
+// A lane's target: position, stream index, whether it is a real point, and
+// the source tiles holding the records of its two adjacent segments.
+struct LaneTarget {
   double r[3];
-  if (valid) {
-    load3(A.P, A.cap, s, r);
-  } else {
-    load3(A.P, A.cap, T * TILE, r);
-  }
-  const int adjA = valid ? A.adj[2 * s] / TILE : -1;
-  const int adjB = valid ? A.adj[2 * s + 1] / TILE : -1;
-  double acc[3] = {0, 0, 0};
+  int s;
+  bool valid;
+  int adjA, adjB;
+};
+
+// Work items [w0, w1) of one target tile: item w = (source tile J_base +
+// w / n_img, image w % n_img).  MIXED: FP32 pass for source tiles beyond the
+// near radius; otherwise all FP64.  Adds the raw sums (without kappa/4pi).
+template <bool MIXED>
+__device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
+                                             const FarRec* __restrict__ far,
+                                             const NearRec* __restrict__ near,
+                                             const ImgDev* __restrict__ img, int n_img, int J_base,
+                                             const TgtTile& tt, const LaneTarget& L, long w0,
+                                             long w1, double rnear2, double acc[3]) {
   for (long w = w0; w < w1; ++w) {
-    const int J = (int)(w / A.n_img);
-    const int I = (int)(w - (long)J * A.n_img);
-    const ImgDev im = A.img[I];
-    const SrcTile sj = A.src[J];
+    const int Jl = (int)(w / n_img);
+    const int I = (int)(w - (long)Jl * n_img);
+    const int J = J_base + Jl;
+    const ImgDev im = img[I];
+    const SrcTile sj = src[J];
     // image target and the target tile's box under the same map
     double rs[3], tlo[3], thi[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      rs[i] = r[i] - im.shift[i];
+      rs[i] = L.r[i] - im.shift[i];
       tlo[i] = tt.lo[i] - im.shift[i];
       thi[i] = tt.hi[i] - im.shift[i];
     }
     if (im.axis >= 0) {
       const int ax = im.axis;
-      rs[ax] = im.wall2 - r[ax];
+      rs[ax] = im.wall2 - L.r[ax];
       double l = im.wall2 - tt.hi[ax], h = im.wall2 - tt.lo[ax];
       tlo[ax] = l;
       thi[ax] = h;
@@ -720,25 +724,49 @@ __device__ __forceinline__ void direct_body(const DirectArgs& A) {
     double v[3] = {0, 0, 0};
     // the primary sum skips each target's adjacent segments: a tile holding
     // one (possibly wrapped, hence far away) goes through the FP64 pass
-    const bool adjacent = im.primary && __builtin_amdgcn_ballot_w64(J == adjA || J == adjB) != 0;
-    if (MIXED && d2 >= A.rnear2 && !adjacent) {
+    const bool adjacent = im.primary && __builtin_amdgcn_ballot_w64(J == L.adjA || J == L.adjB) != 0;
+    if (MIXED && d2 >= rnear2 && !adjacent) {
       float off[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) off[i] = (float)(sj.c[i] - rs[i]);
-      far_tile(A.far, A.near, sj, J * TILE, off, rs, v);
+      far_tile(far, near, sj, J * TILE, off, rs, v);
     } else {
-      near_tile(A.near, sj, J * TILE, rs, s, im.primary != 0, v);
+      near_tile(near, sj, J * TILE, rs, L.s, im.primary != 0, v);
     }
     if (im.axis >= 0) v[im.axis] = -v[im.axis];
     acc[0] += v[0];
     acc[1] += v[1];
     acc[2] += v[2];
   }
-  if (valid) {
+}
+
+// Grid: blockIdx.x = group of 4 target tiles (one wavefront each),
+// blockIdx.y = chunk of the (source tile, image) work list.  Each wave writes
+// its chunk's sum to partial[chunk]; k_reduce adds the chunks in order.
+template <bool MIXED>
+__device__ __forceinline__ void direct_body(const DirectArgs& A) {
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int T = blockIdx.x * 4 + wid;
+  if (T >= A.n_tgt_tiles) return;
+  const int chunk = blockIdx.y;
+  const long n_work = (long)A.counts[1] * A.n_img;
+  const long w0 = (n_work * chunk) / A.n_chunks;
+  const long w1 = (n_work * (chunk + 1)) / A.n_chunks;
+  LaneTarget L;
+  L.s = T * TILE + lane;
+  L.valid = L.s < A.n;
+  load3(A.P, A.cap, L.valid ? L.s : T * TILE, L.r);
+  L.adjA = L.valid ? A.adj[2 * L.s] / TILE : -1;
+  L.adjB = L.valid ? A.adj[2 * L.s + 1] / TILE : -1;
+  const TgtTile tt = A.tgt[T];
+  double acc[3] = {0, 0, 0};
+  direct_items<MIXED>(A.src, A.far, A.near, A.img, A.n_img, 0, tt, L, w0, w1, A.rnear2, acc);
+  if (L.valid) {
     double* out = A.partial + (long)chunk * 3 * A.cap;
-    out[s] = acc[0] * A.kappa4pi;
-    out[A.cap + s] = acc[1] * A.kappa4pi;
-    out[2 * A.cap + s] = acc[2] * A.kappa4pi;
+    out[L.s] = acc[0] * A.kappa4pi;
+    out[A.cap + L.s] = acc[1] * A.kappa4pi;
+    out[2 * A.cap + L.s] = acc[2] * A.kappa4pi;
   }
 }
 
@@ -764,15 +792,14 @@ __global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ parti
 //   vs   = v_LIA + vs_ext + v_BS
 //   vels = vs - [|s''| > c/dl_max] alpha_h s' x v_LIA
 //             + alpha s' x (vn - vs) - alpha' s' x (s' x (vn - vs))
-__global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ T,
-                                                  const double* __restrict__ Nn,
-                                                  const double* __restrict__ lia,
-                                                  const double* __restrict__ bs, int have_bs, int n,
-                                                  long cap, PhysDev ph, double* __restrict__ vs_out,
-                                                  double* __restrict__ vel_out,
-                                                  int* __restrict__ flag) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+__device__ __forceinline__ void epilogue_point(int s, const double* __restrict__ T,
+                                               const double* __restrict__ Nn,
+                                               const double* __restrict__ lia,
+                                               const double* __restrict__ bs, int have_bs,
+                                               long cap, const PhysDev& ph,
+                                               double* __restrict__ vs_out,
+                                               double* __restrict__ vel_out,
+                                               int* __restrict__ flag) {
   double t[3], nm[3], l[3], vs[3], v[3];
   load3(T, cap, s, t);
   load3(Nn, cap, s, nm);
@@ -812,6 +839,18 @@ __global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ T,
   bool bad = !(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]) && isfinite(vs[0]) &&
                isfinite(vs[1]) && isfinite(vs[2]));
   if (bad) atomicOr(flag, 1);
+}
+
+__global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ T,
+                                                  const double* __restrict__ Nn,
+                                                  const double* __restrict__ lia,
+                                                  const double* __restrict__ bs, int have_bs, int n,
+                                                  long cap, PhysDev ph, double* __restrict__ vs_out,
+                                                  double* __restrict__ vel_out,
+                                                  int* __restrict__ flag) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  epilogue_point(s, T, Nn, lia, bs, have_bs, cap, ph, vs_out, vel_out, flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -858,10 +897,8 @@ __global__ void __launch_bounds__(256) k_combine(double* __restrict__ X0,
 // Periodic wrap (reference enforce_boundaries, tangle.c:766-819): while the
 // point is beyond a periodic face (the last offending face in x, y, z order),
 // move it by one box length.  Non-finite points are left alone.
-__global__ void __launch_bounds__(256) k_wrap(double* __restrict__ X0, int n, long cap, BoxDev box,
-                                              int wall_periodic_mask) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+__device__ __forceinline__ void wrap_point(int s, double* __restrict__ X0, long cap,
+                                           const BoxDev& box, int wall_periodic_mask) {
   double x[3];
   load3(X0, cap, s, x);
   if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) return;
@@ -880,6 +917,13 @@ __global__ void __launch_bounds__(256) k_wrap(double* __restrict__ X0, int n, lo
       x[ax] += box.L[ax];
   }
   store3(X0, cap, s, x);
+}
+
+__global__ void __launch_bounds__(256) k_wrap(double* __restrict__ X0, int n, long cap, BoxDev box,
+                                              int wall_periodic_mask) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  wrap_point(s, X0, cap, box, wall_periodic_mask);
 }
 
 }  // namespace ov

@@ -32,6 +32,8 @@ SYMBOLS = (
     "ov_ctx_last_interactions",
     "ov_ctx_last_device_ms",
     "ov_ctx_tree_stats",
+    "ov_gpu_rk4_run_batch",
+    "ov_ctx_batch_tuning",
 )
 
 _lib = None
@@ -81,6 +83,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.ov_ctx_last_device_ms.restype = C.c_double
     lib.ov_ctx_tree_stats.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double)]
     lib.ov_ctx_tree_stats.restype = None
+    lib.ov_gpu_rk4_run_batch.argtypes = [C.c_void_p, P(P(TangleState)), C.c_int, C.c_double, C.c_double,
+                                         C.c_int, P(Params), C.c_int]
+    lib.ov_gpu_rk4_run_batch.restype = C.c_int
+    lib.ov_ctx_batch_tuning.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_double)]
+    lib.ov_ctx_batch_tuning.restype = None
     _lib = lib
     return lib
 
@@ -135,6 +142,24 @@ class Context:
         flags = (OV_ENFORCE_BOUNDARIES if enforce_boundaries else 0) | (OV_PROFILE if profile else 0)
         rc = self.lib.ov_gpu_rk4_run(self.ptr, C.byref(ts), t0, dt, nsteps, C.byref(params), flags)
         return self._check(rc, allow_nonfinite)
+
+    def rk4_run_batch(self, tangles, t0: float, dt: float, nsteps: int, params,
+                      enforce_boundaries=False, allow_nonfinite=False) -> int:
+        """nsteps RK4 steps of independent tangles in one launch per stage;
+        params is one Params or a list (one per tangle)."""
+        if isinstance(params, Params):
+            params = [params] * len(tangles)
+        structs = [t.cstruct() for t in tangles]
+        arr = (C.POINTER(TangleState) * len(tangles))(*[C.pointer(x) for x in structs])
+        parr = (Params * len(tangles))(*params)
+        flags = OV_ENFORCE_BOUNDARIES if enforce_boundaries else 0
+        rc = self.lib.ov_gpu_rk4_run_batch(self.ptr, arr, len(tangles), t0, dt, nsteps, parr, flags)
+        return self._check(rc, allow_nonfinite)
+
+    def batch_tuning(self):
+        u, b, ms = C.c_int(), C.c_int(), C.c_double()
+        self.lib.ov_ctx_batch_tuning(self.ptr, C.byref(u), C.byref(b), C.byref(ms))
+        return u.value, b.value, ms.value
 
     # -- profiling -----------------------------------------------------------
     def kernel_stats(self, name: str):

@@ -432,3 +432,28 @@ def kelvin_ring(seed=1004, npts=4096, r=0.01, modes=range(2, 9), amp_hi=2e-4) ->
     b = TangleBuilder()
     b.add_loop(pts)
     return b.build()
+
+
+# He II mutual-friction coefficients (alpha, alpha') at 1.3, 1.6, 1.9 K, the
+# rows of the reference's python/tcal/mutual_friction.csv at those temperatures.
+MUTUAL_FRICTION = {1.3: (0.034, 1.383e-2), 1.6: (0.097, 1.608e-2), 1.9: (0.206, 8.340e-3)}
+
+
+def ensemble(n_tangles=256, seed=1003, box=0.05, n_lo=500, n_hi=20000, spacing=2e-4,
+             r_lo=0.005, r_hi=0.02):
+    """Independent periodic tangles: tangle i uses seed seed * 2**32 + i; its
+    point target is log-uniform in [n_lo, n_hi], filled with seeded rings as
+    random_rings; its counterflow vn = (U[0.1, 1.0], 0, 0) cm/s and its
+    temperature one of 1.3/1.6/1.9 K (alpha, alpha' from MUTUAL_FRICTION).
+    Returns [(tangle, kwargs for default_params)]."""
+    out = []
+    for i in range(n_tangles):
+        rng = SplitMix64(seed * 2 ** 32 + i)
+        target = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        vn = rng.uniform(0.1, 1.0)
+        temp = (1.3, 1.6, 1.9)[int(rng.uniform(0, 3)) % 3]
+        t = random_rings(seed * 2 ** 32 + i + 7, box, n_points_target=target, r_lo=r_lo, r_hi=r_hi,
+                         spacing=spacing)
+        a, ap = MUTUAL_FRICTION[temp]
+        out.append((t, dict(alpha=a, alpha_p=ap, vn=(vn, 0.0, 0.0), temperature=temp)))
+    return out
