@@ -19,6 +19,9 @@ update_velocity (oracle/_ref, or the FP64 restatement where that is absent)
 on a seeded sample of targets of the same tangle, on this host's cores.
 The Barnes-Hut octree (theta 0.3/0.5/0.7, FP64 and mixed) is evaluated on the
 same positions and compared with the FP64 direct sum on 4096 seeded targets.
+An ensemble of 256 independent periodic tangles (box 0.05 cm, 500-20000
+points each) is stepped in one launch per stage and, for comparison, one
+tangle at a time.
 """
 from __future__ import annotations
 
@@ -63,6 +66,9 @@ def main():
     ap.add_argument("--fp64", type=int, default=1, help="FP64 evaluations for the comparison (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=64, help="targets of the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--ensemble", type=int, default=256,
+                    help="tangles of the batched-ensemble measurement (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=2)
     ap.add_argument("--tree", type=int, default=1,
                     help="octree evaluations at theta 0.3/0.5/0.7 against the FP64 direct sum (0: skip)")
     args = ap.parse_args()
@@ -165,6 +171,41 @@ def main():
                              "err_vels_vs_fp64_direct_sample": err_stats(x.vels[samp], b.vels[samp])})
                 log(f"tree {pname} theta {th}: {ms:.1f} ms, err {tree[-1]['err_vels_vs_fp64_direct_sample']}")
         out["octree"] = tree
+
+    # ---- ensemble: independent tangles, one launch per stage vs a loop -------
+    if args.ensemble > 0:
+        ens = T.ensemble(n_tangles=args.ensemble)
+
+        def ep(kw):
+            kw = dict(kw)
+            kw.pop("temperature")
+            return T.default_params(vortex_width=1.3e-8, hyperfriction=0, precision=T.PREC_MIXED,
+                                    near_radius=args.near_radius, **kw)
+        ps = [ep(kw) for _, kw in ens]
+        ts = [t.copy() for t, _ in ens]
+        ctx.rk4_run_batch(ts, 0.0, 5e-6, 0, ps)          # layout + tuner, no step
+        ts = [t.copy() for t, _ in ens]
+        ctx.rk4_run_batch(ts, 0.0, 5e-6, args.ensemble_steps, ps)
+        b_ms = ctx.last_device_ms()
+        U, blocks, tune_ms = ctx.batch_tuning()
+        loop_ms = 0.0
+        ts2 = [t.copy() for t, _ in ens]
+        for x, pp in zip(ts2, ps):
+            ctx.rk4_run(x, 0.0, 5e-6, args.ensemble_steps, pp)
+            loop_ms += ctx.last_device_ms()
+        npts = sum(len(t.active) for t, _ in ens)
+        inter = sum(27.0 * len(t.active) ** 2 for t, _ in ens)
+        out["ensemble"] = {
+            "tangles": args.ensemble, "points": npts, "steps": args.ensemble_steps,
+            "batched_ms_per_step": round(b_ms / args.ensemble_steps, 2),
+            "loop_ms_per_step": round(loop_ms / args.ensemble_steps, 2),
+            "aggregate_steps_per_s_batched": args.ensemble * args.ensemble_steps / (b_ms / 1e3),
+            "aggregate_steps_per_s_loop": args.ensemble * args.ensemble_steps / (loop_ms / 1e3),
+            "interactions_per_s_batched": 4 * inter * args.ensemble_steps / (b_ms / 1e3),
+            "tuned_unit_items": U, "tuned_blocks": blocks, "tuning_ms": round(tune_ms, 1),
+            "max_position_diff_batched_vs_loop": float(max(np.abs(a.vnodes - c.vnodes).max()
+                                                         for a, c in zip(ts, ts2)))}
+        log(f"ensemble: {out['ensemble']}")
 
     # ---- CPU baseline: the reference's update_velocity on sampled targets ---
     cpu = None
