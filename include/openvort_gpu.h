@@ -173,6 +173,30 @@ int ov_gpu_rk4_run_batch(ov_ctx *ctx, ov_tangle_state *const *tangles, int ntang
  * the tuning time of the last batch layout. */
 void ov_ctx_batch_tuning(const ov_ctx *ctx, int *unit, int *blocks, double *tune_ms);
 
+/* Adjoint of update_tangle + update_velocities (tangle.c:583-702, geometry
+ * tangle.c:704-727): grad_vnodes = (d vels / d vnodes)^T cot_vels at the
+ * tangle's positions, FP64, direct sum (use_BH must be 0; precision selects
+ * the forward sum whose values enter the mutual-friction terms).  Both arrays
+ * have the tangle's capacity N; EMPTY entries of grad_vnodes are set to 0.
+ * Flags: OV_PROFILE.  No counterpart in the reference (it has no gradients). */
+int ov_gpu_velocity_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t,
+                        const ov_params *params, const ov_vec3d *cot_vels,
+                        ov_vec3d *grad_vnodes, int flags);
+
+/* Adjoint of ov_gpu_rk4_run from the tangle's vnodes (each step
+ * update_tangle + rk4_step, vortex_dynamics.c:58-121, in the params' RK
+ * variant): grad_initial = (d final vnodes / d initial vnodes)^T cot_final.
+ * The forward trajectory is recomputed on the device, keeping the positions
+ * every checkpoint_every steps (<= 0: only the initial ones) and, for the
+ * segment being differentiated, every step's stage data (8 N x 24 bytes per
+ * step).  OV_ENFORCE_BOUNDARIES wraps as the forward did (the wrap's
+ * derivative is the identity); OV_USE_CALLER_K1 is rejected.  EMPTY points
+ * do not move, so their entries of grad_initial are cot_final's.  The tangle
+ * is not modified.  No counterpart in the reference. */
+int ov_gpu_rk4_run_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t0, double dt,
+                       int nsteps, const ov_params *params, int flags, int checkpoint_every,
+                       const ov_vec3d *cot_final, ov_vec3d *grad_initial);
+
 /* Per-kernel timings gathered while OV_PROFILE was set; kernel names are the
  * device function names.  Returns the number of launches, 0 if none. */
 long ov_ctx_kernel_stats(ov_ctx *ctx, const char *kernel, double *total_ms);

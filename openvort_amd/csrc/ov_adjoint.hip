@@ -1,0 +1,414 @@
+// ov_adjoint.hip — vector-Jacobian products (VJPs) of the point velocity and
+// of the RK4 step, FP64, direct sum, open / periodic / mirror images.
+//
+// The velocity of point k is vels_k = F(p_{k-2..k+2}, u_k) with u_k the
+// Biot-Savart sum at p_k (reference update_velocity, tangle.c:583-684):
+//   * local part F — stencil geometry (tangle.c:245-273, 432-479), LIA
+//     (tangle.c:530-551), hyperfriction and mutual friction (tangle.c:641-675)
+//     — differentiated per point with dual numbers (value + 3 seeded
+//     directions; 6 seedings: the five stencil points and u_k); the five
+//     transposed blocks are gathered back to the points they came from;
+//   * the segment law (tangle.c:486-514) summed over segments and images:
+//     for one pair, with R = a - r, R' = b - r, l = |R|, l' = |R'|,
+//     d = R.R', q = l l' + d, g = l l' q, f = (l + l') / g, c = R x R',
+//     B = f c, and for a cotangent w
+//       w^T dB/dR  = f (R' x w) + (w.c) (f_l R / l  + f_d R')
+//       w^T dB/dR' = f (w x R)  + (w.c) (f_l' R' / l' + f_d R)
+//     with f_l = (1 - f l' (l l' + q)) / g, f_l' = (1 - f l (l l' + q)) / g,
+//     f_d = -f l l' / g.  The target gets -(dB/dR + dB/dR'), the segment's
+//     start dB/dR and its (minimum-image) end dB/dR'.  Pairs the reference
+//     zeroes (l or l' < 1e-8, |R.R'/(l l') - 1| < 1e-8) contribute nothing.
+// Branch points (the minimum image, the hyperfriction switch) are
+// differentiated on the side the values select.
+#include <hip/hip_runtime.h>
+
+#include "ov_device.h"
+
+namespace ov {
+
+struct D3 {   // value and derivatives along 3 seeded directions
+  double v, d[3];
+};
+__device__ inline D3 dc(double x) { return {x, {0, 0, 0}}; }
+__device__ inline D3 operator+(const D3& a, const D3& b) {
+  return {a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]}};
+}
+__device__ inline D3 operator-(const D3& a, const D3& b) {
+  return {a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]}};
+}
+__device__ inline D3 operator-(const D3& a) { return {-a.v, {-a.d[0], -a.d[1], -a.d[2]}}; }
+__device__ inline D3 operator*(const D3& a, const D3& b) {
+  return {a.v * b.v,
+          {a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]}};
+}
+__device__ inline D3 operator*(const D3& a, double s) {
+  return {a.v * s, {a.d[0] * s, a.d[1] * s, a.d[2] * s}};
+}
+__device__ inline D3 operator*(double s, const D3& a) { return a * s; }
+__device__ inline D3 operator/(const D3& a, const D3& b) {
+  const double q = a.v / b.v;
+  return {q, {(a.d[0] - q * b.d[0]) / b.v, (a.d[1] - q * b.d[1]) / b.v, (a.d[2] - q * b.d[2]) / b.v}};
+}
+__device__ inline D3 dsqrt(const D3& a) {
+  const double r = sqrt(a.v), h = 0.5 / r;
+  return {r, {a.d[0] * h, a.d[1] * h, a.d[2] * h}};
+}
+__device__ inline D3 dlog(const D3& a) { return {log(a.v), {a.d[0] / a.v, a.d[1] / a.v, a.d[2] / a.v}}; }
+
+__device__ inline void dcross(const D3 a[3], const D3 b[3], D3 o[3]) {
+  const D3 x = a[1] * b[2] - a[2] * b[1];
+  const D3 y = a[2] * b[0] - a[0] * b[2];
+  const D3 z = a[0] * b[1] - a[1] * b[0];
+  o[0] = x;
+  o[1] = y;
+  o[2] = z;
+}
+
+// minimum image of r2 seen from r1 (the shift is decided on values)
+__device__ inline void dpwrap(const D3 r1[3], const D3 r2[3], const BoxDev& b, D3 out[3]) {
+  for (int i = 0; i < 3; ++i) {
+    D3 o = r2[i];
+    if (b.periodic[i]) {
+      const double d = r2[i].v - r1[i].v;
+      if (d > b.L[i] / 2)
+        o.v -= b.L[i];
+      else if (d < -b.L[i] / 2)
+        o.v += b.L[i];
+    }
+    out[i] = o;
+  }
+}
+
+__device__ inline D3 dchord(const D3 s[3], const D3 sp[3], const BoxDev& b) {
+  D3 w[3];
+  dpwrap(s, sp, b, w);
+  const D3 d0 = s[0] - w[0], d1 = s[1] - w[1], d2 = s[2] - w[2];
+  return dsqrt(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+// Local part of the point velocity (the same arithmetic as geometry_point +
+// epilogue_point): p[0..4] = positions of k-2, k-1, k, k+1, k+2, u = the
+// Biot-Savart sum at k.
+__device__ void local_velocity(const D3 p[5][3], const D3 u[3], const BoxDev& box, const PhysDev& ph,
+                               D3 vel[3]) {
+  const D3 dk = dchord(p[2], p[3], box), df = dchord(p[3], p[4], box);
+  const D3 dr = dchord(p[1], p[2], box), drr = dchord(p[0], p[1], box);
+  const D3 x[4] = {dk + df, dk, -dr, -(dr + drr)};
+  const int nbi[4] = {4, 3, 1, 0};
+  D3 t[3] = {dc(0), dc(0), dc(0)}, nm[3] = {dc(0), dc(0), dc(0)};
+  for (int j = 0; j < 4; ++j) {
+    D3 o[3];
+    int q = 0;
+    for (int m = 0; m < 4; ++m)
+      if (m != j) o[q++] = x[m];
+    const D3 den = x[j] * (x[j] - o[0]) * (x[j] - o[1]) * (x[j] - o[2]);
+    const D3 w1 = -(o[0] * o[1] * o[2]) / den;
+    const D3 w2 = 2.0 * (o[0] * o[1] + o[0] * o[2] + o[1] * o[2]) / den;
+    D3 s0w[3];
+    dpwrap(p[nbi[j]], p[2], box, s0w);
+    for (int i = 0; i < 3; ++i) {
+      const D3 d = p[nbi[j]][i] - s0w[i];
+      t[i] = t[i] + w1 * d;
+      nm[i] = nm[i] + w2 * d;
+    }
+  }
+  const double pi = 3.14159265358979323846;
+  const D3 beta =
+      dlog(dsqrt(dk * dr) * (2.0 / sqrt(sqrt(2.718281828459045)) / ph.a0)) * (ph.kappa / 4 / pi);
+  D3 lia[3];
+  dcross(t, nm, lia);
+  for (int i = 0; i < 3; ++i) lia[i] = lia[i] * beta;
+  D3 vs[3];
+  for (int i = 0; i < 3; ++i) vs[i] = lia[i] + dc(ph.vs_ext[i]) + u[i];
+  for (int i = 0; i < 3; ++i) vel[i] = vs[i];
+  if (ph.hyperfriction) {
+    const double spp = sqrt(nm[0].v * nm[0].v + nm[1].v * nm[1].v + nm[2].v * nm[2].v);
+    if (spp > ph.max_curv / ph.dl_max) {
+      D3 h[3];
+      dcross(t, lia, h);
+      for (int i = 0; i < 3; ++i) vel[i] = vel[i] - h[i] * ph.hyperalpha;
+    }
+  }
+  if (ph.use_mf) {
+    D3 dv[3], a1[3], a2[3];
+    for (int i = 0; i < 3; ++i) dv[i] = dc(ph.vn_ext[i]) - vs[i];
+    dcross(t, dv, a1);
+    dcross(t, a1, a2);
+    for (int i = 0; i < 3; ++i) vel[i] = vel[i] + a1[i] * ph.alpha - a2[i] * ph.alpha_p;
+  }
+}
+
+// Local VJP of every point k: gnb[j][k] = (dvel_k/dp_j)^T w_k for the five
+// stencil slots j (k-2, k-1, k, k+1, k+2), gu[k] = (dvel_k/du_k)^T w_k.
+__global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
+                                                const int* __restrict__ fwd,
+                                                const int* __restrict__ rev,
+                                                const double* __restrict__ bs, int have_bs,
+                                                const double* __restrict__ w, int n, long cap,
+                                                BoxDev box, PhysDev ph, double* __restrict__ gnb,
+                                                double* __restrict__ gu) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int ids[5];
+  ids[2] = k;
+  ids[3] = fwd[k];
+  ids[4] = fwd[ids[3]];
+  ids[1] = rev[k];
+  ids[0] = rev[ids[1]];
+  double wk[3], uk[3] = {0, 0, 0};
+  load3(w, cap, k, wk);
+  if (have_bs) load3(bs, cap, k, uk);
+  for (int seed = 0; seed < 6; ++seed) {
+    D3 p[5][3], u[3];
+    for (int j = 0; j < 5; ++j)
+      for (int i = 0; i < 3; ++i) {
+        p[j][i] = dc(P[i * cap + ids[j]]);
+        if (seed == j) p[j][i].d[i] = 1.0;
+      }
+    for (int i = 0; i < 3; ++i) {
+      u[i] = dc(uk[i]);
+      if (seed == 5) u[i].d[i] = 1.0;
+    }
+    D3 vel[3];
+    local_velocity(p, u, box, ph, vel);
+    double g[3];
+    for (int c = 0; c < 3; ++c) g[c] = vel[0].d[c] * wk[0] + vel[1].d[c] * wk[1] + vel[2].d[c] * wk[2];
+    if (seed < 5) {
+      for (int c = 0; c < 3; ++c) gnb[(long)(seed * 3 + c) * cap + k] = g[c];
+    } else {
+      store3(gu, cap, k, g);
+    }
+  }
+}
+
+// gl[p] = the local VJP blocks naming p: p is slot 0 of k+2, slot 1 of k+1,
+// slot 2 of itself, slot 3 of k-1 and slot 4 of k-2.
+__global__ void __launch_bounds__(256) ka_gather(const double* __restrict__ gnb,
+                                                 const int* __restrict__ fwd,
+                                                 const int* __restrict__ rev, int n, long cap,
+                                                 double* __restrict__ gl) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int f = fwd[p], r = rev[p];
+  const int owner[5] = {fwd[f], f, p, r, rev[r]};
+  double g[3] = {0, 0, 0};
+  for (int j = 0; j < 5; ++j)
+    for (int c = 0; c < 3; ++c) g[c] += gnb[(long)(j * 3 + c) * cap + owner[j]];
+  store3(gl, cap, p, g);
+}
+
+// SB[m] = minimum image of the segment end fwd[m] seen from m
+__global__ void __launch_bounds__(256) ka_seg_end(const double* __restrict__ P,
+                                                  const int* __restrict__ fwd, int n, long cap,
+                                                  BoxDev box, double* __restrict__ SB) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double a[3], f[3], b[3];
+  load3(P, cap, m, a);
+  load3(P, cap, fwd[m], f);
+  pwrap(a, f, box, b);
+  store3(SB, cap, m, b);
+}
+
+// VJP of one pair: gR = w^T dB/dR, gRp = w^T dB/dR' (B without kappa/4pi).
+// Returns false (and nothing) for pairs the reference zeroes.
+__device__ __forceinline__ bool pair_vjp(const double a[3], const double b[3], const double rs[3],
+                                         const double w[3], double gR[3], double gRp[3]) {
+  double R[3], Rp[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    R[i] = a[i] - rs[i];
+    Rp[i] = b[i] - rs[i];
+  }
+  const double l2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+  const double lp2 = Rp[0] * Rp[0] + Rp[1] * Rp[1] + Rp[2] * Rp[2];
+  const double d = R[0] * Rp[0] + R[1] * Rp[1] + R[2] * Rp[2];
+  // zero rules: a cheap estimate; pairs near a threshold take the exact test.
+  // Lengths are compared squared first: a segment end can coincide with an
+  // image target (a wrapped segment seen from the image that undoes the wrap).
+  if (l2 < 1.000002e-16 || lp2 < 1.000002e-16) {
+    if (guard_exact(a, b, rs)) return false;
+  }
+  const double il = rsq64(l2), ilp = rsq64(lp2);
+  const double l = l2 * il, lp = lp2 * ilp;
+  if (fabs(d * il * ilp - 1.0) < 1.000001e-8) {
+    if (guard_exact(a, b, rs)) return false;
+  }
+  const double ll = l * lp, q = ll + d, g = ll * q, ig = rcp64(g);
+  const double f = (l + lp) * ig;
+  const double fl = ig * (1.0 - f * lp * (ll + q)) * il;   // f_l / l
+  const double flp = ig * (1.0 - f * l * (ll + q)) * ilp;  // f_l' / l'
+  const double fd = -f * ll * ig;
+  const double c[3] = {R[1] * Rp[2] - R[2] * Rp[1], R[2] * Rp[0] - R[0] * Rp[2],
+                       R[0] * Rp[1] - R[1] * Rp[0]};
+  const double wc = w[0] * c[0] + w[1] * c[1] + w[2] * c[2];
+  const double pw[3] = {Rp[1] * w[2] - Rp[2] * w[1], Rp[2] * w[0] - Rp[0] * w[2],
+                        Rp[0] * w[1] - Rp[1] * w[0]};   // R' x w
+  const double wr[3] = {w[1] * R[2] - w[2] * R[1], w[2] * R[0] - w[0] * R[2],
+                        w[0] * R[1] - w[1] * R[0]};     // w x R
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    gR[i] = f * pw[i] + wc * (fl * R[i] + fd * Rp[i]);
+    gRp[i] = f * wr[i] + wc * (flp * Rp[i] + fd * R[i]);
+  }
+  return true;
+}
+
+// image target and image cotangent: rs = r - shift, mirrored on the image's
+// axis (rs[ax] = 2 wall - r[ax]); the image velocity is mirrored on that axis,
+// so the cotangent is too.
+__device__ __forceinline__ void image_map(const ImgDev& im, const double r[3], const double w[3],
+                                          double rs[3], double wi[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    rs[i] = r[i] - im.shift[i];
+    wi[i] = w[i];
+  }
+  if (im.axis >= 0) {
+    rs[im.axis] = im.wall2 - r[im.axis];
+    wi[im.axis] = -w[im.axis];
+  }
+}
+
+struct AdjArgs {
+  const double* P;    // positions (SoA, stride cap)
+  const double* SB;   // minimum-image segment ends
+  const int* fwd;
+  const double* gu;   // cotangent of each point's Biot-Savart sum (without kappa/4pi)
+  const ImgDev* img;
+  long cap;
+  int n;
+  int n_tiles;
+  int n_img;
+  int n_chunks;
+  double* part;       // targets: [chunks][3][cap]; segments: [chunks][6][cap]
+};
+
+// Targets per lane, (segment, image) pairs wave-uniform: partial sums of
+// sum_{m, images} (dB/dr)^T gu_k over a chunk of segments.
+__global__ void __launch_bounds__(256) ka_bs_targets(AdjArgs A) {
+  const int lane = threadIdx.x & 63;
+  const int T = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (T >= A.n_tiles) return;
+  const int chunk = blockIdx.y;
+  const int m0 = (int)(((long)A.n * chunk) / A.n_chunks);
+  const int m1 = (int)(((long)A.n * (chunk + 1)) / A.n_chunks);
+  const int k = T * TILE + lane;
+  const bool valid = k < A.n;
+  const int kk = valid ? k : T * TILE;
+  double r[3], w[3], g[3] = {0, 0, 0};
+  load3(A.P, A.cap, kk, r);
+  load3(A.gu, A.cap, kk, w);
+  for (int m = m0; m < m1; ++m) {
+    double a[3], b[3];
+    load3(A.P, A.cap, m, a);
+    load3(A.SB, A.cap, m, b);
+    const int fm = A.fwd[m];
+    const bool adjacent = m == kk || fm == kk;
+    for (int I = 0; I < A.n_img; ++I) {
+      const ImgDev im = A.img[I];
+      if (im.primary && adjacent) continue;
+      double rs[3], wi[3], gR[3], gRp[3];
+      image_map(im, r, w, rs, wi);
+      if (!pair_vjp(a, b, rs, wi, gR, gRp)) continue;
+      double gr[3] = {-(gR[0] + gRp[0]), -(gR[1] + gRp[1]), -(gR[2] + gRp[2])};
+      if (im.axis >= 0) gr[im.axis] = -gr[im.axis];
+      g[0] += gr[0];
+      g[1] += gr[1];
+      g[2] += gr[2];
+    }
+  }
+  if (valid) store3(A.part + (long)chunk * 3 * A.cap, A.cap, k, g);
+}
+
+// Segments per lane, (target, image) pairs wave-uniform: partial sums of
+// sum_{k, images} (dB/da)^T gu_k and (dB/db)^T gu_k over a chunk of targets.
+__global__ void __launch_bounds__(256) ka_bs_sources(AdjArgs A) {
+  const int lane = threadIdx.x & 63;
+  const int T = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (T >= A.n_tiles) return;
+  const int chunk = blockIdx.y;
+  const int k0 = (int)(((long)A.n * chunk) / A.n_chunks);
+  const int k1 = (int)(((long)A.n * (chunk + 1)) / A.n_chunks);
+  const int m = T * TILE + lane;
+  const bool valid = m < A.n;
+  const int mm = valid ? m : T * TILE;
+  double a[3], b[3], ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+  load3(A.P, A.cap, mm, a);
+  load3(A.SB, A.cap, mm, b);
+  const int fm = A.fwd[mm];
+  for (int k = k0; k < k1; ++k) {
+    double r[3], w[3];
+    load3(A.P, A.cap, k, r);
+    load3(A.gu, A.cap, k, w);
+    const bool adjacent = mm == k || fm == k;
+    for (int I = 0; I < A.n_img; ++I) {
+      const ImgDev im = A.img[I];
+      if (im.primary && adjacent) continue;
+      double rs[3], wi[3], gR[3], gRp[3];
+      image_map(im, r, w, rs, wi);
+      if (!pair_vjp(a, b, rs, wi, gR, gRp)) continue;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        ga[i] += gR[i];
+        gb[i] += gRp[i];
+      }
+    }
+  }
+  if (valid) {
+    double* out = A.part + (long)chunk * 6 * A.cap;
+    store3(out, A.cap, m, ga);
+    store3(out + 3 * A.cap, A.cap, m, gb);
+  }
+}
+
+// G[p] = gl[p] + kappa/4pi (sum_c tpart_c[p] + sum_c ga_c[p] + sum_c gb_c[rev p])
+// (p starts segment p and ends segment rev p); chunks added in order.
+__global__ void __launch_bounds__(256) ka_bs_finish(const double* __restrict__ gl,
+                                                    const double* __restrict__ tpart,
+                                                    const double* __restrict__ spart, int n_chunks,
+                                                    const int* __restrict__ rev, int n, long cap,
+                                                    double kappa4pi, double* __restrict__ G) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int r = rev[p];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double t = 0, sa = 0, sb = 0;
+    for (int c = 0; c < n_chunks; ++c) {
+      t += tpart[(long)(3 * c + i) * cap + p];
+      sa += spart[(long)(6 * c + i) * cap + p];
+      sb += spart[(long)(6 * c + 3 + i) * cap + r];
+    }
+    G[i * cap + p] = gl[i * cap + p] + kappa4pi * (t + (sa + sb));
+  }
+}
+
+// y = x + a z (y may be x)
+__global__ void __launch_bounds__(256) ka_axpy(const double* x, double a,
+                                               const double* __restrict__ z, double* y, int n,
+                                               long cap) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[c * cap + s] = x[c * cap + s] + a * z[c * cap + s];
+}
+
+// y = a x
+__global__ void __launch_bounds__(256) ka_scale(const double* __restrict__ x, double a,
+                                                double* __restrict__ y, int n, long cap) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[c * cap + s] = a * x[c * cap + s];
+}
+
+// flag bit 4: a non-finite gradient entry
+__global__ void __launch_bounds__(256) ka_finite(const double* __restrict__ G, int n, long cap,
+                                                 int* __restrict__ flag) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  if (!(isfinite(G[s]) && isfinite(G[cap + s]) && isfinite(G[2 * cap + s]))) atomicOr(flag, 4);
+}
+
+}  // namespace ov

@@ -21,6 +21,7 @@
 #include "ov_kernels.hip"
 #include "ov_octree.hip"
 #include "ov_batch.hip"
+#include "ov_adjoint.hip"
 
 #include <hipcub/hipcub.hpp>
 
@@ -122,6 +123,13 @@ struct ov_ctx {
   std::map<std::pair<long, int>, std::pair<int, int>> b_tuned;   // (ntot, K) -> (U, blocks)
   int b_U = 256, b_blocks = 1024;
   double b_tune_ms = 0;
+
+  // adjoint (planes of `cap` doubles: gnb 15, the others 3)
+  double *a_gnb = nullptr, *a_gu = nullptr, *a_gl = nullptr, *a_SB = nullptr, *a_W = nullptr,
+         *a_G = nullptr, *a_Gtmp = nullptr, *a_gK[4] = {nullptr, nullptr, nullptr, nullptr};
+  long a_cap = 0;
+  double *a_tpart = nullptr, *a_spart = nullptr, *a_ck = nullptr, *a_store = nullptr;
+  size_t a_tpart_bytes = 0, a_spart_bytes = 0, a_ck_bytes = 0, a_store_bytes = 0;
 };
 
 namespace {
@@ -773,25 +781,43 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
   return OV_OK;
 }
 
+// Copy of one vec3 plane set (3 planes of cap doubles) on the context stream.
+int copy_planes(ov_ctx* ctx, double* dst, const double* src) {
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * 3 * (size_t)ctx->cap, hipMemcpyDeviceToDevice,
+                        ctx->stream));
+  return OV_OK;
+}
+
 // update_tangle(t) + rk4_step(t, dt) on the device-resident state in X0.
 // With caller_k1, K[0] already holds k1 and the t-geometry is not produced.
-int rk4_device(ov_ctx* ctx, double t, double dt, int rk_variant, bool caller_k1) {
+// With `store` (8 plane sets), the step's start positions, k1..k3 and the
+// Biot-Savart sums of the four evaluations are kept there, in the order
+// X0, K0, K1, K2, BS0, BS1, BS2, BS3 (what the step's VJP needs).
+int rk4_device(ov_ctx* ctx, double t, double dt, int rk_variant, bool caller_k1,
+               double* store = nullptr) {
   const int n = ctx->n;
   const long cap = ctx->cap;
+  const long ps = 3 * cap;
   int rc;
+  if (store && (rc = copy_planes(ctx, store, ctx->X0))) return rc;
   if (!caller_k1) {
     if ((rc = evaluate(ctx, ctx->X0, true, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0])))
       return rc;
+    if (store && (rc = copy_planes(ctx, store + 4 * ps, ctx->bs))) return rc;
   }
   const double h[3] = {dt / 2, dt / 2, rk_variant == OV_RK_CLASSIC ? dt : dt / 2};
-  const double ts[3] = {t + dt / 2, t + dt / 2, t + dt};
+  (void)t;
   for (int st = 0; st < 3; ++st) {
     if ((rc = launch(ctx, "k_stage", k_stage, grid1(n), dim3(256), (const double*)ctx->X0,
                      (const double*)ctx->K[st], h[st], n, cap, ctx->Xs)))
       return rc;
-    (void)ts;
     if ((rc = evaluate(ctx, ctx->Xs, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[st + 1])))
       return rc;
+    if (store && (rc = copy_planes(ctx, store + (5 + st) * ps, ctx->bs))) return rc;
+  }
+  if (store) {
+    for (int st = 0; st < 3; ++st)
+      if ((rc = copy_planes(ctx, store + (1 + st) * ps, ctx->K[st]))) return rc;
   }
   return launch(ctx, "k_combine", k_combine, grid1(n), dim3(256), ctx->X0, (const double*)ctx->K[0],
                 (const double*)ctx->K[1], (const double*)ctx->K[2], (const double*)ctx->K[3], dt, n,
@@ -937,6 +963,130 @@ int rk4_batch(ov_ctx* ctx, double dt, int rk_variant, bool enforce) {
   return OV_OK;
 }
 
+
+// ---- adjoint ------------------------------------------------------------------
+
+int grow_bytes(ov_ctx* ctx, double** p, size_t* have, size_t need) {
+  if (need <= *have) return OV_OK;
+  int rc = dalloc(ctx, (void**)p, need);
+  *have = rc ? 0 : need;
+  return rc;
+}
+
+int adj_buffers(ov_ctx* ctx) {
+  if (ctx->a_cap >= ctx->cap) return OV_OK;
+  const size_t v3 = sizeof(double) * 3 * (size_t)ctx->cap;
+  int rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->a_gnb, 5 * v3))) return rc;
+  double** b[] = {&ctx->a_gu, &ctx->a_gl,    &ctx->a_SB,    &ctx->a_W,     &ctx->a_G,
+                  &ctx->a_Gtmp, &ctx->a_gK[0], &ctx->a_gK[1], &ctx->a_gK[2], &ctx->a_gK[3]};
+  for (double** p : b)
+    if ((rc = dalloc(ctx, (void**)p, v3))) return rc;
+  ctx->a_cap = ctx->cap;
+  return OV_OK;
+}
+
+// G = (dvels/dX)^T W at positions X whose Biot-Savart sums are `bs`.
+int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const double* W, double* G) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  const int have_bs = ctx->lia_only ? 0 : 1;
+  int rc;
+  if ((rc = launch(ctx, "ka_local", ka_local, dim3((unsigned)((n + 127) / 128)), dim3(128), X,
+                   (const int*)ctx->fwd, (const int*)ctx->rev, bs, have_bs, W, n, cap, ctx->box,
+                   ctx->ph, ctx->a_gnb, ctx->a_gu)))
+    return rc;
+  if ((rc = launch(ctx, "ka_gather", ka_gather, grid1(n), dim3(256), (const double*)ctx->a_gnb,
+                   (const int*)ctx->fwd, (const int*)ctx->rev, n, cap, ctx->a_gl)))
+    return rc;
+  if (!have_bs)
+    return launch(ctx, "ka_scale", ka_scale, grid1(n), dim3(256), (const double*)ctx->a_gl, 1.0, G,
+                  n, cap);
+  if ((rc = launch(ctx, "ka_seg_end", ka_seg_end, grid1(n), dim3(256), X, (const int*)ctx->fwd, n,
+                   cap, ctx->box, ctx->a_SB)))
+    return rc;
+  const int n_tiles = (n + TILE - 1) / TILE;
+  // enough (tile, chunk) waves to fill 256 CUs
+  int chunks = (8192 + n_tiles - 1) / n_tiles;
+  chunks = std::max(1, std::min(chunks, std::min(256, n)));
+  if ((rc = grow_bytes(ctx, &ctx->a_tpart, &ctx->a_tpart_bytes, sizeof(double) * 3 * cap * chunks)) ||
+      (rc = grow_bytes(ctx, &ctx->a_spart, &ctx->a_spart_bytes, sizeof(double) * 6 * cap * chunks)))
+    return rc;
+  AdjArgs A;
+  A.P = X;
+  A.SB = ctx->a_SB;
+  A.fwd = ctx->fwd;
+  A.gu = ctx->a_gu;
+  A.img = ctx->img;
+  A.cap = cap;
+  A.n = n;
+  A.n_tiles = n_tiles;
+  A.n_img = (int)ctx->imgs.size();
+  A.n_chunks = chunks;
+  const dim3 grid((unsigned)((n_tiles + 3) / 4), (unsigned)chunks);
+  A.part = ctx->a_tpart;
+  if ((rc = launch(ctx, "ka_bs_targets", ka_bs_targets, grid, dim3(256), A))) return rc;
+  A.part = ctx->a_spart;
+  if ((rc = launch(ctx, "ka_bs_sources", ka_bs_sources, grid, dim3(256), A))) return rc;
+  return launch(ctx, "ka_bs_finish", ka_bs_finish, grid1(n), dim3(256), (const double*)ctx->a_gl,
+                (const double*)ctx->a_tpart, (const double*)ctx->a_spart, chunks,
+                (const int*)ctx->rev, n, cap, ctx->ph.kappa / 4 / 3.14159265358979323846, G);
+}
+
+// On entry G = cotangent of the step's end positions; on return the cotangent
+// of its start positions.  `store` is what rk4_device kept for the step.
+int step_vjp_dev(ov_ctx* ctx, const double* store, double dt, int rk_variant, double* G) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  const long ps = 3 * cap;
+  const double h[3] = {dt / 2, dt / 2, rk_variant == OV_RK_CLASSIC ? dt : dt / 2};
+  const double wk[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  int rc;
+  for (int i = 0; i < 4; ++i)
+    if ((rc = launch(ctx, "ka_scale", ka_scale, grid1(n), dim3(256), (const double*)G, wk[i],
+                     ctx->a_gK[i], n, cap)))
+      return rc;
+  // stage st + 1 was evaluated at X0 + h[st] k_{st+1}
+  for (int st = 2; st >= -1; --st) {
+    const double* Xs = store;
+    if (st >= 0) {
+      if ((rc = launch(ctx, "k_stage", k_stage, grid1(n), dim3(256), store, store + (1 + st) * ps,
+                       h[st], n, cap, ctx->Xs)))
+        return rc;
+      Xs = ctx->Xs;
+    }
+    if ((rc = velocity_vjp_dev(ctx, Xs, store + (5 + st) * ps, ctx->a_gK[st + 1], ctx->a_Gtmp)))
+      return rc;
+    if ((rc = launch(ctx, "ka_axpy", ka_axpy, grid1(n), dim3(256), (const double*)G, 1.0,
+                     (const double*)ctx->a_Gtmp, G, n, cap)))
+      return rc;
+    if (st >= 0 && (rc = launch(ctx, "ka_axpy", ka_axpy, grid1(n), dim3(256),
+                                (const double*)ctx->a_gK[st], h[st], (const double*)ctx->a_Gtmp,
+                                ctx->a_gK[st], n, cap)))
+      return rc;
+  }
+  return OV_OK;
+}
+
+// Common checks of the adjoint entries.
+int adj_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
+  int rc = prepare(ctx, t, p);
+  if (rc) return rc;
+  if (p->use_BH)
+    return fail(ctx, OV_E_UNSUPPORTED, "the adjoint differentiates the direct sum only (use_BH=0)");
+  if (ctx->n == 0) return OV_OK;
+  if ((rc = adj_buffers(ctx))) return rc;
+  return upload_topology(ctx);
+}
+
+// Fill every caller entry with `pass` (or 0), then scatter the active points'
+// gradient over it.
+int download_grad(ov_ctx* ctx, const ov_tangle_state* t, const double* G, const ov_vec3d* pass,
+                  ov_vec3d* grad) {
+  for (int k = 0; k < t->N; ++k) grad[k] = pass ? pass[k] : ov_vec3d{{0, 0, 0}};
+  return download_vec3(ctx, G, grad);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1004,6 +1154,11 @@ void ov_ctx_destroy(ov_ctx* ctx) {
                 ctx->b_tS,   ctx->b_ucnt,  ctx->b_urank,       ctx->b_ubsum,      ctx->b_ucounts,
                 ctx->b_queue, ctx->b_img,  ctx->b_upart};
   for (void* b : bb)
+    if (b) hipFree(b);
+  void* ab[] = {ctx->a_gnb,   ctx->a_gu,    ctx->a_gl,    ctx->a_SB,    ctx->a_W,
+                ctx->a_G,     ctx->a_Gtmp,  ctx->a_gK[0], ctx->a_gK[1], ctx->a_gK[2],
+                ctx->a_gK[3], ctx->a_tpart, ctx->a_spart, ctx->a_ck,    ctx->a_store};
+  for (void* b : ab)
     if (b) hipFree(b);
   if (ctx->hst) hipHostFree(ctx->hst);
   for (auto& p : ctx->pending) {
@@ -1383,6 +1538,98 @@ int ov_gpu_rk4_step(ov_ctx* ctx, ov_tangle_state* t, double time, double dt, con
 int ov_gpu_rk4_run(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps,
                    const ov_params* p, int flags) {
   return rk4_common(ctx, t, t0, dt, nsteps, p, flags);
+}
+
+
+int ov_gpu_velocity_vjp(ov_ctx* ctx, const ov_tangle_state* t, double time, const ov_params* p,
+                        const ov_vec3d* cot_vels, ov_vec3d* grad_vnodes, int flags) {
+  if (!ctx) return OV_E_INVALID;
+  if (!cot_vels || !grad_vnodes) return fail(ctx, OV_E_INVALID, "null cotangent or gradient");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = adj_prepare(ctx, t, p);
+  if (rc) return rc;
+  if (ctx->n == 0) {
+    for (int k = 0; k < t->N; ++k) grad_vnodes[k] = ov_vec3d{{0, 0, 0}};
+    return OV_OK;
+  }
+  ctx->profile = (flags & OV_PROFILE) != 0;
+  (void)time;
+  if ((rc = upload_vec3(ctx, t->vnodes, ctx->X0)) || (rc = upload_vec3(ctx, cot_vels, ctx->a_W)))
+    return rc;
+  if ((rc = mark_begin(ctx))) return rc;
+  if ((rc = evaluate(ctx, ctx->X0, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[0])) ||
+      (rc = velocity_vjp_dev(ctx, ctx->X0, ctx->bs, ctx->a_W, ctx->a_G)) ||
+      (rc = launch(ctx, "ka_finite", ka_finite, grid1(ctx->n), dim3(256), (const double*)ctx->a_G,
+                   ctx->n, ctx->cap, ctx->flag)))
+    return rc;
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  if ((rc = download_grad(ctx, t, ctx->a_G, nullptr, grad_vnodes))) return rc;
+  if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite %s", (flag & 4) ? "gradient" : "velocity");
+  return OV_OK;
+}
+
+int ov_gpu_rk4_run_vjp(ov_ctx* ctx, const ov_tangle_state* t, double t0, double dt, int nsteps,
+                       const ov_params* p, int flags, int checkpoint_every,
+                       const ov_vec3d* cot_final, ov_vec3d* grad_initial) {
+  if (!ctx) return OV_E_INVALID;
+  if (!cot_final || !grad_initial) return fail(ctx, OV_E_INVALID, "null cotangent or gradient");
+  if (nsteps < 0) return fail(ctx, OV_E_INVALID, "negative step count");
+  if (flags & OV_USE_CALLER_K1)
+    return fail(ctx, OV_E_UNSUPPORTED, "the adjoint recomputes k1 from the positions");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = adj_prepare(ctx, t, p);
+  if (rc) return rc;
+  if (ctx->n == 0 || nsteps == 0) {   // nothing moves: the map is the identity
+    for (int k = 0; k < t->N; ++k) grad_initial[k] = cot_final[k];
+    return OV_OK;
+  }
+  ctx->profile = (flags & OV_PROFILE) != 0;
+  const bool enforce = (flags & OV_ENFORCE_BOUNDARIES) != 0;
+  const int n = ctx->n;
+  const long ps = 3 * ctx->cap;
+  const int k = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
+  const int n_seg = (nsteps + k - 1) / k;
+  if ((rc = grow_bytes(ctx, &ctx->a_ck, &ctx->a_ck_bytes, sizeof(double) * ps * n_seg)) ||
+      (rc = grow_bytes(ctx, &ctx->a_store, &ctx->a_store_bytes, sizeof(double) * ps * 8 * k)))
+    return rc;
+  if ((rc = upload_vec3(ctx, t->vnodes, ctx->X0)) || (rc = upload_vec3(ctx, cot_final, ctx->a_G)))
+    return rc;
+  if ((rc = mark_begin(ctx))) return rc;
+  auto wrap = [&]() {
+    return enforce ? launch(ctx, "k_wrap", k_wrap, grid1(n), dim3(256), ctx->X0, n, ctx->cap,
+                            ctx->box, ctx->wall_periodic_mask)
+                   : OV_OK;
+  };
+  // forward sweep keeping the start of every segment (one segment: the input)
+  if ((rc = copy_planes(ctx, ctx->a_ck, ctx->X0))) return rc;
+  for (int s = 0; s < (n_seg - 1) * k; ++s) {
+    if ((rc = rk4_device(ctx, t0 + s * dt, dt, p->rk_variant, false)) || (rc = wrap())) return rc;
+    if ((s + 1) % k == 0 && (rc = copy_planes(ctx, ctx->a_ck + ((s + 1) / k) * ps, ctx->X0)))
+      return rc;
+  }
+  // reverse sweep: recompute a segment keeping every step, then its VJPs
+  for (int seg = n_seg - 1; seg >= 0; --seg) {
+    const int s0 = seg * k, s1 = std::min(nsteps, s0 + k);
+    if ((rc = copy_planes(ctx, ctx->X0, ctx->a_ck + seg * ps))) return rc;
+    for (int s = s0; s < s1; ++s)
+      if ((rc = rk4_device(ctx, t0 + s * dt, dt, p->rk_variant, false, ctx->a_store + (s - s0) * 8 * ps)) ||
+          (rc = wrap()))
+        return rc;
+    for (int s = s1 - 1; s >= s0; --s)
+      if ((rc = step_vjp_dev(ctx, ctx->a_store + (s - s0) * 8 * ps, dt, p->rk_variant, ctx->a_G)))
+        return rc;
+  }
+  if ((rc = launch(ctx, "ka_finite", ka_finite, grid1(n), dim3(256), (const double*)ctx->a_G, n,
+                   ctx->cap, ctx->flag)))
+    return rc;
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  if ((rc = download_grad(ctx, t, ctx->a_G, cot_final, grad_initial))) return rc;
+  if (flag)
+    return fail(ctx, OV_E_NONFINITE, "non-finite %s",
+                (flag & 4) ? "gradient" : (flag & 2) ? "position" : "velocity");
+  return OV_OK;
 }
 
 }  // extern "C"

@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 from .tangle import Params, Tangle, TangleState
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -34,6 +36,8 @@ SYMBOLS = (
     "ov_ctx_tree_stats",
     "ov_gpu_rk4_run_batch",
     "ov_ctx_batch_tuning",
+    "ov_gpu_velocity_vjp",
+    "ov_gpu_rk4_run_vjp",
 )
 
 _lib = None
@@ -88,6 +92,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.ov_gpu_rk4_run_batch.restype = C.c_int
     lib.ov_ctx_batch_tuning.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_double)]
     lib.ov_ctx_batch_tuning.restype = None
+    V3 = C.c_void_p   # ov_vec3d* as a numpy (N, 3) float64 buffer
+    lib.ov_gpu_velocity_vjp.argtypes = [C.c_void_p, P(TangleState), C.c_double, P(Params), V3, V3, C.c_int]
+    lib.ov_gpu_velocity_vjp.restype = C.c_int
+    lib.ov_gpu_rk4_run_vjp.argtypes = [C.c_void_p, P(TangleState), C.c_double, C.c_double, C.c_int,
+                                       P(Params), C.c_int, C.c_int, V3, V3]
+    lib.ov_gpu_rk4_run_vjp.restype = C.c_int
     _lib = lib
     return lib
 
@@ -155,6 +165,37 @@ class Context:
         flags = OV_ENFORCE_BOUNDARIES if enforce_boundaries else 0
         rc = self.lib.ov_gpu_rk4_run_batch(self.ptr, arr, len(tangles), t0, dt, nsteps, parr, flags)
         return self._check(rc, allow_nonfinite)
+
+    @staticmethod
+    def _vec3(a, n):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (n, 3):
+            raise ValueError(f"expected an ({n}, 3) array, got {a.shape}")
+        return a
+
+    def velocity_vjp(self, t: Tangle, time: float, params: Params, cot_vels, profile=False):
+        """(d vels / d vnodes)^T cot_vels at t's positions; (N, 3) arrays
+        indexed like the tangle (EMPTY rows of the result are 0)."""
+        w = self._vec3(cot_vels, t.N)
+        g = np.zeros((t.N, 3))
+        ts = t.cstruct()
+        rc = self.lib.ov_gpu_velocity_vjp(self.ptr, C.byref(ts), time, C.byref(params),
+                                          w.ctypes.data, g.ctypes.data, OV_PROFILE if profile else 0)
+        self._check(rc)
+        return g
+
+    def rk4_run_vjp(self, t: Tangle, t0: float, dt: float, nsteps: int, params: Params, cot_final,
+                    checkpoint_every=0, enforce_boundaries=False, profile=False):
+        """(d final vnodes / d initial vnodes)^T cot_final for nsteps RK4
+        steps from t's positions (t is not modified)."""
+        w = self._vec3(cot_final, t.N)
+        g = np.zeros((t.N, 3))
+        ts = t.cstruct()
+        flags = (OV_ENFORCE_BOUNDARIES if enforce_boundaries else 0) | (OV_PROFILE if profile else 0)
+        rc = self.lib.ov_gpu_rk4_run_vjp(self.ptr, C.byref(ts), t0, dt, nsteps, C.byref(params), flags,
+                                         checkpoint_every, w.ctypes.data, g.ctypes.data)
+        self._check(rc)
+        return g
 
     def batch_tuning(self):
         u, b, ms = C.c_int(), C.c_int(), C.c_double()
