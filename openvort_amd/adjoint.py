@@ -41,6 +41,7 @@ class Run:
     nsteps: int
     checkpoint_every: int = 20
     enforce_boundaries: bool = False
+    profile: bool = False   # per-kernel HIP-event timings of the adjoint (Context.kernel_stats)
 
 
 def _with_positions(t: T.Tangle, X: torch.Tensor) -> T.Tangle:
@@ -69,7 +70,7 @@ class RK4Run(torch.autograd.Function):
         gx = run.ctx.rk4_run_vjp(t, run.t0, run.dt, run.nsteps, run.params,
                                  g.detach().to("cpu", torch.float64).numpy(),
                                  checkpoint_every=run.checkpoint_every,
-                                 enforce_boundaries=run.enforce_boundaries)
+                                 enforce_boundaries=run.enforce_boundaries, profile=run.profile)
         return torch.from_numpy(gx).to(g.device), None
 
 

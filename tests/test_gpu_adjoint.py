@@ -49,6 +49,18 @@ def test_velocity_vjp_matches_autograd(ctx, name):
     assert np.all(g[empty] == 0)
 
 
+def test_velocity_vjp_many_tiles_matches_autograd(ctx):
+    """1024 points: 16 tiles, 128 chunks per pass (the small cases fit one tile)."""
+    t = T.kelvin_ring(npts=1024)
+    p = T.default_params(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, hyperfriction=0,
+                         vs=(0.0, 0.01, 0.0), vn=(0.2, 0.0, 0.0))
+    w = AC.seeded((t.N, 3), 14)
+    g = ctx.velocity_vjp(t, 0.0, p, w)
+    e = _err(g, _ref_velocity_vjp(t, p, w))
+    print("1024-point vjp vs autograd", e, flush=True)
+    assert e < AD_TOL
+
+
 @pytest.mark.parametrize("name", ["ring_mf_flows", "periodic_holes", "mirror"])
 def test_velocity_vjp_matches_oracle_fd(ctx, oracle, name):
     t, p = AC.case(name)
@@ -187,3 +199,44 @@ def test_velocity_function_backward(ctx):
     (A.velocity(X, ctx, t, p) * w).sum().backward()
     g = ctx.velocity_vjp(t, 0.0, p, w.numpy())
     assert np.array_equal(X.grad.numpy(), g)
+
+
+def test_kelvin_ring_full_size_gradient_and_fd_mechanism(ctx):
+    """The differentiation workload's ring (4096 points) over 5 steps at
+    dt = 1e-7: the GPU adjoint against autograd through the torch
+    restatement run on the GPU (the FP64 reference gradient).  Central
+    differences in a random direction are biased here by the reference's
+    collinearity cut-off (|R.R'/(l l') - 1| < 1e-8 zeroes a pair,
+    src/tangle.c:505-506): ~1e5 pairs sit near it, a perturbation flips
+    O(h) of them and each flip jumps the velocity, so the bias does not
+    shrink with h.  With the cut-off set to 0 in the restatement, its
+    central differences match its autograd gradient."""
+    dev = torch.device("cuda:0")
+    t = T.kelvin_ring()
+    p = T.default_params(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, hyperfriction=0)
+    dt, nsteps = 1e-7, 5
+    w = AC.seeded((t.N, 3), 15)
+    d = torch.from_numpy(AC.seeded((t.N, 3), 16)).to(dev)
+    g = ctx.rk4_run_vjp(t, 0.0, dt, nsteps, p, w, checkpoint_every=2)
+    wt = torch.from_numpy(w).to(dev)
+    for tol in (1e-8, 0.0):
+        topo = TR.Topo(t, p)
+        topo.ndot_tol = tol
+        X = torch.from_numpy(t.vnodes.copy()).to(dev).requires_grad_(True)
+        (TR.rk4_run(X, topo, dt, nsteps) * wt).sum().backward()
+        ref = X.grad.cpu().numpy()
+        if tol > 0:
+            e = _err(g, ref)
+            print("4096-point adjoint vs autograd", e, flush=True)
+            assert e < AD_TOL
+        else:
+            X0 = X.detach()
+            ad = float((X.grad * d).sum())
+            with torch.no_grad():
+                f = lambda Y: float((TR.rk4_run(Y, topo, dt, nsteps) * wt).sum())
+                h = 1e-9
+                fd = (f(X0 + h * d) - f(X0 - h * d)) / (2 * h)
+            print("cut-off off: fd", fd, "ad", ad, flush=True)
+            assert abs(fd - ad) <= FD_TOL * abs(ad)
+        del X
+        torch.cuda.empty_cache()

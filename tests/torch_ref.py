@@ -39,6 +39,9 @@ class Topo:
         for sx, sy, sz, rf in t.images:
             self.images.append(((self.L[0] * sx, self.L[1] * sy, self.L[2] * sz), rf, False))
         self.p = p
+        # collinearity cut-off of the segment law (src/tangle.c:505-506); the
+        # reference's 1e-8 unless a check sets it otherwise
+        self.ndot_tol = 1e-8
 
 
 def pwrap(r1, r2, topo):
@@ -58,7 +61,7 @@ def pwrap(r1, r2, topo):
 
 def geometry(X, topo):
     """(s', s'', l+, l-) of every point."""
-    f, r = topo.fwd, topo.rev
+    f, r = topo.fwd.to(X.device), topo.rev.to(X.device)
     p0, pf, pff, pr, prr = X, X[f], X[f[f]], X[r], X[r[r]]
     chord = lambda a, b: torch.linalg.norm(a - pwrap(a, b, topo), dim=1)
     dk, df = chord(p0, pf), chord(pf, pff)
@@ -83,14 +86,15 @@ def biot_savart(X, topo):
     p = topo.p
     n = X.shape[0]
     a = X
-    b = pwrap(a, X[topo.fwd], topo)
-    idx = torch.arange(n)
-    adjacent = (idx[:, None] == idx[None, :]) | (topo.fwd[None, :] == idx[:, None])
+    fwd = topo.fwd.to(X.device)
+    b = pwrap(a, X[fwd], topo)
+    idx = torch.arange(n, device=X.device)
+    adjacent = (idx[:, None] == idx[None, :]) | (fwd[None, :] == idx[:, None])
     out = torch.zeros_like(X)
-    one = torch.tensor([1.0, 0.0, 0.0], dtype=X.dtype)
-    two = torch.tensor([0.0, 1.0, 0.0], dtype=X.dtype)
+    one = torch.tensor([1.0, 0.0, 0.0], dtype=X.dtype, device=X.device)
+    two = torch.tensor([0.0, 1.0, 0.0], dtype=X.dtype, device=X.device)
     for shift, rf, primary in topo.images:
-        rs = X - torch.tensor(shift, dtype=X.dtype)
+        rs = X - torch.tensor(shift, dtype=X.dtype, device=X.device)
         if rf >= 0:
             ax = rf // 2
             wall = topo.hi[ax] if rf & 1 else topo.lo[ax]
@@ -102,7 +106,7 @@ def biot_savart(X, topo):
             l = torch.linalg.norm(R, dim=2)
             lp = torch.linalg.norm(Rp, dim=2)
             nd = (R * Rp).sum(2) / l / lp
-            zero = (l < 1e-8) | (lp < 1e-8) | ((nd - 1).abs() < 1e-8)
+            zero = (l < 1e-8) | (lp < 1e-8) | ((nd - 1).abs() < topo.ndot_tol)
             if primary:
                 zero = zero | adjacent
         keep = ~zero
@@ -127,8 +131,10 @@ def velocities(X, topo):
     t, nm, dk, dr = geometry(X, topo)
     beta = p.kappa / 4 / math.pi * torch.log(2 * torch.sqrt(dk * dr) / math.exp(0.25) / p.vortex_width)
     lia = beta[:, None] * torch.cross(t, nm, dim=1)
-    vs_ext = torch.tensor(list(p.vs.v) if p.vs.type == T.FLOW_SIMPLE else [0.0] * 3, dtype=X.dtype)
-    vn_ext = torch.tensor(list(p.vn.v) if p.vn.type == T.FLOW_SIMPLE else [0.0] * 3, dtype=X.dtype)
+    vs_ext = torch.tensor(list(p.vs.v) if p.vs.type == T.FLOW_SIMPLE else [0.0] * 3, dtype=X.dtype,
+                          device=X.device)
+    vn_ext = torch.tensor(list(p.vn.v) if p.vn.type == T.FLOW_SIMPLE else [0.0] * 3, dtype=X.dtype,
+                          device=X.device)
     vs = lia + vs_ext
     if not p.LIA_only:
         vs = vs + biot_savart(X, topo)
