@@ -191,6 +191,7 @@ __global__ void __launch_bounds__(64) kb_src_tiles(NearRec* __restrict__ near,
     near[e] = pad;
   }
   far[e] = fr;
+  const double dmax2 = wave_max((double)(fr.d[0] * fr.d[0] + fr.d[1] * fr.d[1] + fr.d[2] * fr.d[2]));
   if (lane == 0) {
     SrcTile t;
 #pragma unroll
@@ -201,7 +202,7 @@ __global__ void __launch_bounds__(64) kb_src_tiles(NearRec* __restrict__ near,
       t.prev[i] = prev[i];
       t.prev_loc[i] = (float)(prev[i] - c[i]);
     }
-    t.pad = 0;
+    t.dmax2 = (float)dmax2;
     tiles[j] = t;
   }
 }

@@ -49,14 +49,14 @@ struct TgtTile {
 // Source tile: centre c of the bounding box of its records (and of the
 // record before it), the FP32 frame origin; the box; the record before the
 // tile (start of a segment closed by the tile's first record), in FP64 and
-// in the tile frame.
+// in the tile frame; the largest squared segment length of its FP32 records.
 struct SrcTile {
   double c[3];
   double lo[3];
   double hi[3];
   double prev[3];
   float prev_loc[3];
-  int pad;
+  float dmax2;
 };
 
 // FP32 far-field record: point relative to the tile centre and the segment

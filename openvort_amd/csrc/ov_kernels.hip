@@ -366,6 +366,7 @@ __global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
     near[e] = pad;
   }
   far[e] = fr;
+  const double dmax2 = wave_max((double)(fr.d[0] * fr.d[0] + fr.d[1] * fr.d[1] + fr.d[2] * fr.d[2]));
   if (lane == 0) {
     SrcTile t;
 #pragma unroll
@@ -376,7 +377,7 @@ __global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
       t.prev[i] = prev[i];
       t.prev_loc[i] = (float)(prev[i] - c[i]);
     }
-    t.pad = 0;
+    t.dmax2 = (float)dmax2;
     tiles[J] = t;
   }
 }
@@ -675,6 +676,128 @@ __device__ __forceinline__ void far_tile(const FarRec* __restrict__ rec,
   }
 }
 
+// Far field at small angles.  Every pair of a source tile whose longest
+// segment subtends at most theta (sin^2 theta <= SMALL_ANGLE2) from all
+// targets has s = sin^2 of the angle between R and R' below SMALL_ANGLE2, and
+//   1 / (1 + cos) = 1/2 + s/8 + s^2/16 + 5 s^3/128 + ...,
+// so f = (u + u') u u' / (1 + cos) needs no reciprocal: the truncation after
+// s^2 is below 5 SMALL_ANGLE2^3 / 64 = 5e-9 relative.  s is the quantity the
+// zero rule already computes (s = |R x d|^2 u^2 u'^2), so the per-pair
+// reciprocal and the R.R' product drop out.
+constexpr float SMALL_ANGLE2 = 4e-3f;
+
+// One FP32 small-angle pair (the FP32 arithmetic of far_tile_fast, written
+// with explicit fusions so that a recomputation gives the same bits).
+struct PairS {
+  float R[3], l2, u;
+  float c[3], f, g;
+};
+__device__ __forceinline__ PairS pair_small(const float Rp[3], float up, const FarRec& q,
+                                            const float off[3]) {
+#pragma clang fp contract(off)
+  PairS o;
+  o.R[0] = q.p[0] + off[0];
+  o.R[1] = q.p[1] + off[1];
+  o.R[2] = q.p[2] + off[2];
+  o.l2 = fmaf(o.R[2], o.R[2], fmaf(o.R[1], o.R[1], o.R[0] * o.R[0]));
+  o.u = __builtin_amdgcn_rsqf(o.l2);
+  o.c[0] = fmaf(Rp[1], q.d[2], -(Rp[2] * q.d[1]));
+  o.c[1] = fmaf(Rp[2], q.d[0], -(Rp[0] * q.d[2]));
+  o.c[2] = fmaf(Rp[0], q.d[1], -(Rp[1] * q.d[0]));
+  const float c2 = fmaf(o.c[2], o.c[2], fmaf(o.c[1], o.c[1], o.c[0] * o.c[0]));
+  const float w = up * o.u;
+  o.g = fmaf(c2, w * w, -2e-8f);   // s - 2e-8: the zero rule's estimate
+  // 1/(1+cos) as a polynomial in g = s - 2e-8
+  constexpr float s0 = 2e-8f;
+  const float h = fmaf(o.g, fmaf(o.g, 0.0625f, 0.125f + 0.125f * s0), 0.5f + 0.125f * s0);
+  o.f = (up + o.u) * w * h;
+  return o;
+}
+
+// FP32 pass over one source tile at small angles (far_tile without the
+// reciprocal).  Zero-rule band detection: each record's band test becomes a
+// wave mask; one branch per 8-record group, and only when some lane of the
+// group is in a band are that group's records re-evaluated (pair_small gives
+// the same bits) to note which lanes were in a band and what they decided.
+__device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
+                                              const NearRec* __restrict__ nrec, const SrcTile& st,
+                                              int base, const float off[3], const double rs[3],
+                                              double acc[3]) {
+  const float R0[3] = {st.prev_loc[0] + off[0], st.prev_loc[1] + off[1], st.prev_loc[2] + off[2]};
+  float Rp[3] = {R0[0], R0[1], R0[2]};
+  float up = __builtin_amdgcn_rsqf(fmaf(Rp[2], Rp[2], fmaf(Rp[1], Rp[1], Rp[0] * Rp[0])));
+  uint64_t bandmask = 0, lane_band = 0, lane_zero = 0;
+  for (int e0 = 0; e0 < TILE; e0 += 8) {
+    FarRec q[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q[k] = rec[base + e0 + k];
+    const float Rg[3] = {Rp[0], Rp[1], Rp[2]};
+    const float ug = up;
+    float a8[3] = {0.f, 0.f, 0.f};
+    uint64_t any = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const PairS o = pair_small(Rp, up, q[k], off);
+      const float f = o.g < 0.f ? 0.f : o.f;
+      a8[0] = fmaf(f, o.c[0], a8[0]);
+      a8[1] = fmaf(f, o.c[1], a8[1]);
+      a8[2] = fmaf(f, o.c[2], a8[2]);
+      any |= __builtin_amdgcn_ballot_w64(fabsf(o.g) < 1e-11f);
+      Rp[0] = o.R[0];
+      Rp[1] = o.R[1];
+      Rp[2] = o.R[2];
+      up = o.u;
+    }
+    acc[0] += (double)a8[0];
+    acc[1] += (double)a8[1];
+    acc[2] += (double)a8[2];
+    if (__builtin_expect(any != 0, 0)) {
+      float Rb[3] = {Rg[0], Rg[1], Rg[2]};
+      float ub = ug;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const PairS o = pair_small(Rb, ub, q[k], off);
+        const bool band = fabsf(o.g) < 1e-11f;
+        if (__builtin_amdgcn_ballot_w64(band) != 0) bandmask |= 1ull << (e0 + k);
+        if (band) lane_band |= 1ull << (e0 + k);
+        if (band && o.g < 0.f) lane_zero |= 1ull << (e0 + k);
+        Rb[0] = o.R[0];
+        Rb[1] = o.R[1];
+        Rb[2] = o.R[2];
+        ub = o.u;
+      }
+    }
+  }
+  while (__builtin_expect(bandmask != 0, 0)) {
+    const int e = __builtin_ctzll(bandmask);
+    bandmask &= bandmask - 1;
+    float P[3];
+    if (e == 0) {
+      P[0] = R0[0];
+      P[1] = R0[1];
+      P[2] = R0[2];
+    } else {
+      const FarRec& qp = rec[base + e - 1];
+      P[0] = qp.p[0] + off[0];
+      P[1] = qp.p[1] + off[1];
+      P[2] = qp.p[2] + off[2];
+    }
+    const float pu = __builtin_amdgcn_rsqf(fmaf(P[2], P[2], fmaf(P[1], P[1], P[0] * P[0])));
+    const PairS o = pair_small(P, pu, rec[base + e], off);
+    if ((lane_band >> e) & 1) {
+      const bool zero32 = (lane_zero >> e) & 1;
+      const double* a = e > 0 ? nrec[base + e - 1].p : st.prev;
+      const bool zero = guard_exact(a, nrec[base + e].p, rs);
+      if (zero != zero32) {
+        const double f = zero ? -(double)o.f : (double)o.f;
+        acc[0] += f * (double)o.c[0];
+        acc[1] += f * (double)o.c[1];
+        acc[2] += f * (double)o.c[2];
+      }
+    }
+  }
+}
+
 // A lane's target: position, stream index, whether it is a real point, and
 // the source tiles holding the records of its two adjacent segments.
 struct LaneTarget {
@@ -729,7 +852,11 @@ __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
       float off[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) off[i] = (float)(sj.c[i] - rs[i]);
-      far_tile(far, near, sj, J * TILE, off, rs, v);
+      // d2 bounds every pair's l^2 from below, so dmax2 / d2 bounds sin^2
+      if ((double)sj.dmax2 <= (double)SMALL_ANGLE2 * d2)
+        far_tile_fast(far, near, sj, J * TILE, off, rs, v);
+      else
+        far_tile(far, near, sj, J * TILE, off, rs, v);
     } else {
       near_tile(near, sj, J * TILE, rs, L.s, im.primary != 0, v);
     }
