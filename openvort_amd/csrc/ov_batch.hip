@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(64) kb_src_tiles(NearRec* __restrict__ near,
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       fr.p[i] = (float)(r.p[i] - c[i]);
-      fr.d[i] = link ? (float)(r.p[i] - pp[i]) : 0.f;
+      fr.d[i] = link ? 0.5f * (float)(r.p[i] - pp[i]) : 0.f;
     }
   } else {
     NearRec pad;
@@ -191,7 +191,8 @@ __global__ void __launch_bounds__(64) kb_src_tiles(NearRec* __restrict__ near,
     near[e] = pad;
   }
   far[e] = fr;
-  const double dmax2 = wave_max((double)(fr.d[0] * fr.d[0] + fr.d[1] * fr.d[1] + fr.d[2] * fr.d[2]));
+  const double dmax2 =
+      4.0 * wave_max((double)(fr.d[0] * fr.d[0] + fr.d[1] * fr.d[1] + fr.d[2] * fr.d[2]));
   if (lane == 0) {
     SrcTile t;
 #pragma unroll

@@ -59,9 +59,11 @@ struct SrcTile {
   float dmax2;
 };
 
-// FP32 far-field record: point relative to the tile centre and the segment
-// vector from the previous record (FP64 difference rounded once; zero when
-// the record closes no segment).
+// FP32 far-field record: point relative to the tile centre and half the
+// segment vector from the previous record (FP64 difference rounded once,
+// then halved exactly; zero when the record closes no segment).  The half
+// vector makes the far pass's cross product R x d/2, so |R x d/2|^2 u^2 u'^2
+// is sin^2/4, the term the small-angle form needs, without an extra scaling.
 struct FarRec {
   float p[3];
   float d[3];

@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       fr.p[i] = (float)(r.p[i] - c[i]);
-      fr.d[i] = link ? (float)(r.p[i] - pp[i]) : 0.f;
+      fr.d[i] = link ? 0.5f * (float)(r.p[i] - pp[i]) : 0.f;
     }
   } else {
     NearRec pad;
@@ -366,7 +366,8 @@ __global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
     near[e] = pad;
   }
   far[e] = fr;
-  const double dmax2 = wave_max((double)(fr.d[0] * fr.d[0] + fr.d[1] * fr.d[1] + fr.d[2] * fr.d[2]));
+  const double dmax2 =
+      4.0 * wave_max((double)(fr.d[0] * fr.d[0] + fr.d[1] * fr.d[1] + fr.d[2] * fr.d[2]));
   if (lane == 0) {
     SrcTile t;
 #pragma unroll
@@ -578,8 +579,9 @@ struct Pair32 {
   float R[3], l2, u;
   float c[3], f, g;
 };
-__device__ __forceinline__ Pair32 pair32(const float Rp[3], float l2p, float up, const FarRec& q,
+__device__ __forceinline__ Pair32 pair32(const float Rp[3], float l2p, float up, const FarRec& qh,
                                          const float off[3]) {
+  const FarRec q = {{qh.p[0], qh.p[1], qh.p[2]}, {2.f * qh.d[0], 2.f * qh.d[1], 2.f * qh.d[2]}};
   Pair32 o;
   o.R[0] = q.p[0] + off[0];
   o.R[1] = q.p[1] + off[1];
@@ -679,18 +681,33 @@ __device__ __forceinline__ void far_tile(const FarRec* __restrict__ rec,
 // Far field at small angles.  Every pair of a source tile whose longest
 // segment subtends at most theta (sin^2 theta <= SMALL_ANGLE2) from all
 // targets has s = sin^2 of the angle between R and R' below SMALL_ANGLE2, and
-//   1 / (1 + cos) = 1/2 + s/8 + s^2/16 + 5 s^3/128 + ...,
-// so f = (u + u') u u' / (1 + cos) needs no reciprocal: the truncation after
-// s^2 is below 5 SMALL_ANGLE2^3 / 64 = 5e-9 relative.  s is the quantity the
-// zero rule already computes (s = |R x d|^2 u^2 u'^2), so the per-pair
-// reciprocal and the R.R' product drop out.
-constexpr float SMALL_ANGLE2 = 4e-3f;
+//   2 / (1 + cos) = 1 + s/4 + s^2/8 + ...,
+// so 2f = 2 (u + u') u u' / (1 + cos) needs no reciprocal: 1 + s/4 is within
+// SMALL_ANGLE2^2 / 8 = 2e-8 relative (within 1e-10 for the 1e-4 cm spacing at
+// the 0.02 cm near radius).  With the half segment vector of FarRec the cross
+// product is c/2 = R x d/2 and |c/2|^2 u^2 u'^2 = s/4, the quantity the zero
+// rule tests (s < 2e-8), so the per-pair reciprocal and R.R' drop out and
+// (2f)(c/2) = f c.  2f = b + b (s/4) with b = (u + u') u u' is rounded once:
+// rounding 1/2 + s/8 instead would truncate the small term to 0 for most far
+// pairs (it lies below half an ulp of 1/2) and bias every pair low.
+constexpr float SMALL_ANGLE2 = 4e-4f;
+
+// v_rsq_f32 is within 1 ulp but not unbiased: its mean relative error over
+// operands spread across binades is -7.4e-9 (measured on gfx950 against the
+// correctly rounded value, whose mean is -5e-12).  f carries three of these
+// reciprocal square roots ((u + u') u u') and, unlike the reciprocal form,
+// nothing compensates them, so the per-pair mean error of 2f (c/2) is
+// -2.654e-8 (measured over 2^20 far pairs at 0.02-0.3 cm).  Summed over the
+// ~1e13 pairs of an evaluation that bias is coherent, so each tile's sum is
+// scaled back by this factor (per-pair rms error, 1.2e-7, is unchanged).
+constexpr double RSQ_BIAS_FIX = 1.0 + 2.654e-8;
 
 // One FP32 small-angle pair (the FP32 arithmetic of far_tile_fast, written
-// with explicit fusions so that a recomputation gives the same bits).
+// with explicit fusions so that a recomputation gives the same bits):
+// c = R_prev x d/2, s4 = s/4, f = 2f.
 struct PairS {
   float R[3], l2, u;
-  float c[3], f, g;
+  float c[3], f, s4;
 };
 __device__ __forceinline__ PairS pair_small(const float Rp[3], float up, const FarRec& q,
                                             const float off[3]) {
@@ -706,19 +723,22 @@ __device__ __forceinline__ PairS pair_small(const float Rp[3], float up, const F
   o.c[2] = fmaf(Rp[0], q.d[1], -(Rp[1] * q.d[0]));
   const float c2 = fmaf(o.c[2], o.c[2], fmaf(o.c[1], o.c[1], o.c[0] * o.c[0]));
   const float w = up * o.u;
-  o.g = fmaf(c2, w * w, -2e-8f);   // s - 2e-8: the zero rule's estimate
-  // 1/(1+cos) as a polynomial in g = s - 2e-8
-  constexpr float s0 = 2e-8f;
-  const float h = fmaf(o.g, fmaf(o.g, 0.0625f, 0.125f + 0.125f * s0), 0.5f + 0.125f * s0);
-  o.f = (up + o.u) * w * h;
+  o.s4 = c2 * (w * w);
+  const float b = (up + o.u) * w;
+  o.f = fmaf(b, o.s4, b);
   return o;
 }
+// zero rule s < 2e-8 and the band re-decided exactly, in units of s/4
+__device__ __forceinline__ bool small_zero(float s4) { return s4 < 5e-9f; }
+__device__ __forceinline__ bool small_band(float s4) { return fabsf(s4 - 5e-9f) < 2.5e-12f; }
 
 // FP32 pass over one source tile at small angles (far_tile without the
 // reciprocal).  Zero-rule band detection: each record's band test becomes a
 // wave mask; one branch per 8-record group, and only when some lane of the
 // group is in a band are that group's records re-evaluated (pair_small gives
 // the same bits) to note which lanes were in a band and what they decided.
+// acc receives this tile's sum only (it starts at zero and is scaled by
+// RSQ_BIAS_FIX at the end).
 __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
                                               const NearRec* __restrict__ nrec, const SrcTile& st,
                                               int base, const float off[3], const double rs[3],
@@ -738,11 +758,11 @@ __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const PairS o = pair_small(Rp, up, q[k], off);
-      const float f = o.g < 0.f ? 0.f : o.f;
+      const float f = small_zero(o.s4) ? 0.f : o.f;
       a8[0] = fmaf(f, o.c[0], a8[0]);
       a8[1] = fmaf(f, o.c[1], a8[1]);
       a8[2] = fmaf(f, o.c[2], a8[2]);
-      any |= __builtin_amdgcn_ballot_w64(fabsf(o.g) < 1e-11f);
+      any |= __builtin_amdgcn_ballot_w64(small_band(o.s4));
       Rp[0] = o.R[0];
       Rp[1] = o.R[1];
       Rp[2] = o.R[2];
@@ -757,10 +777,10 @@ __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const PairS o = pair_small(Rb, ub, q[k], off);
-        const bool band = fabsf(o.g) < 1e-11f;
+        const bool band = small_band(o.s4);
         if (__builtin_amdgcn_ballot_w64(band) != 0) bandmask |= 1ull << (e0 + k);
         if (band) lane_band |= 1ull << (e0 + k);
-        if (band && o.g < 0.f) lane_zero |= 1ull << (e0 + k);
+        if (band && small_zero(o.s4)) lane_zero |= 1ull << (e0 + k);
         Rb[0] = o.R[0];
         Rb[1] = o.R[1];
         Rb[2] = o.R[2];
@@ -796,6 +816,9 @@ __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
       }
     }
   }
+  acc[0] *= RSQ_BIAS_FIX;
+  acc[1] *= RSQ_BIAS_FIX;
+  acc[2] *= RSQ_BIAS_FIX;
 }
 
 // A lane's target: position, stream index, whether it is a real point, and
