@@ -55,6 +55,23 @@ def err_stats(x, y):
             "max_abs_over_rms": float(d.max() / np.sqrt(np.mean(n * n)))}
 
 
+def measured_traffic(points, seed, near_radius):
+    """HBM bytes per launch of k_direct_mixed from the committed PMC summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), when it covers
+    this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "k_direct_mixed_bh1m_pmc.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        d = json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return None
+    w = d.get("workload", {})
+    if w.get("points") != points or w.get("seed") != seed or w.get("near_radius_cm") != near_radius:
+        return None
+    return d["hbm_bytes_per_launch"]
+
+
 def adjoint_section(ctx, T):
     """Gradient of the Kelvin-ring loss (centroid target + Kelvin-wave energy
     of the final state) with respect to the initial positions and the mode
@@ -161,7 +178,20 @@ def main():
                     help="Kelvin-ring gradient (4096 points, 200 steps, checkpoints every 20; 0: skip)")
     ap.add_argument("--tree", type=int, default=1,
                     help="octree evaluations at theta 0.3/0.5/0.7 against the FP64 direct sum (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device (HBM-traffic counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = 256 * 1024 * 1024          # 1 GiB of float32
+        x = torch.ones(n, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no "
@@ -349,7 +379,8 @@ def main():
                    "interactions_per_eval": I_eval},
         "rk4_steps_per_s": 1e3 / ms_per_step,
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
+                     "traffic": measured_traffic(n, args.seed, args.near_radius),
                      "kernel": "k_direct_mixed", "avg_launch_ms": round(kd_avg_s * 1e3, 2),
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "cpu_baseline": cpu,

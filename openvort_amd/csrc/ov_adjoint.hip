@@ -362,26 +362,36 @@ __global__ void __launch_bounds__(256) ka_bs_sources(AdjArgs A) {
   }
 }
 
-// G[p] = gl[p] + kappa/4pi (sum_c tpart_c[p] + sum_c ga_c[p] + sum_c gb_c[rev p])
-// (p starts segment p and ends segment rev p); chunks added in order.
-__global__ void __launch_bounds__(256) ka_bs_finish(const double* __restrict__ gl,
-                                                    const double* __restrict__ tpart,
+// red[q][p] (9 planes, q = blockIdx.y): the chunk sums of the target pass
+// (q = 0..2) and of the segment pass (q = 3..5: start, 6..8: end), in chunk
+// order.  red reuses the local pass's gnb planes, free once ka_gather ran.
+__global__ void __launch_bounds__(256) ka_bs_reduce(const double* __restrict__ tpart,
                                                     const double* __restrict__ spart, int n_chunks,
+                                                    int n, long cap, double* __restrict__ red) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (p >= n) return;
+  const double* src = q < 3 ? tpart + (long)q * cap + p : spart + (long)(q - 3) * cap + p;
+  const long stride = (q < 3 ? 3 : 6) * cap;
+  double a = 0;
+#pragma unroll 8
+  for (int c = 0; c < n_chunks; ++c) a += src[c * stride];
+  red[(long)q * cap + p] = a;
+}
+
+// G[p] = gl[p] + kappa/4pi (targets[p] + (starts[p] + ends[rev p]))
+// (p starts segment p and ends segment rev p).
+__global__ void __launch_bounds__(256) ka_bs_finish(const double* __restrict__ gl,
+                                                    const double* __restrict__ red,
                                                     const int* __restrict__ rev, int n, long cap,
                                                     double kappa4pi, double* __restrict__ G) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int r = rev[p];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    double t = 0, sa = 0, sb = 0;
-    for (int c = 0; c < n_chunks; ++c) {
-      t += tpart[(long)(3 * c + i) * cap + p];
-      sa += spart[(long)(6 * c + i) * cap + p];
-      sb += spart[(long)(6 * c + 3 + i) * cap + r];
-    }
-    G[i * cap + p] = gl[i * cap + p] + kappa4pi * (t + (sa + sb));
-  }
+  for (int i = 0; i < 3; ++i)
+    G[i * cap + p] = gl[i * cap + p] +
+                     kappa4pi * (red[i * cap + p] + (red[(3 + i) * cap + p] + red[(6 + i) * cap + r]));
 }
 
 // y = x + a z (y may be x)

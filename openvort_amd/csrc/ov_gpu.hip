@@ -681,7 +681,7 @@ int tree_sum(ov_ctx* ctx, const double* X) {
   else
     rc = launch(ctx, "k_octree_fp64", k_octree_fp64, grid, dim3(256), A);
   if (rc) return rc;
-  if ((rc = launch(ctx, "k_reduce", k_reduce, grid1(n), dim3(256), (const double*)ctx->partial, n_img, n,
+  if ((rc = launch(ctx, "k_reduce", k_reduce, dim3(grid1(n).x, 3), dim3(256), (const double*)ctx->partial, n_img, n,
                    cap, ctx->bs)))
     return rc;
   if (ctx->profile) {
@@ -734,9 +734,11 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
     if ((rc = launch(ctx, "k_adjacent", k_adjacent, dim3((unsigned)((2 * n + 255) / 256)), dim3(256),
                      (const NearRec*)ctx->near, (const int*)ctx->counts, ctx->adj)))
       return rc;
-    // enough (target tile, chunk) waves to fill 256 CUs several times over
+    // enough (target tile, chunk) waves to fill 256 CUs several times over,
+    // but no more chunks than work items per target tile (each chunk's
+    // partial plane costs 24 bytes per point to write and to reduce)
     long chunks = (65536 + n_tgt - 1) / n_tgt;
-    chunks = std::max<long>(1, std::min<long>(chunks, 256));
+    chunks = std::max<long>(1, std::min<long>(chunks, std::min<long>(256, (long)n_tgt * n_img)));
     size_t pbytes = sizeof(double) * 3 * (size_t)cap * (size_t)chunks;
     if (pbytes > ctx->partial_bytes) {
       if ((rc = dalloc(ctx, (void**)&ctx->partial, pbytes))) return rc;
@@ -765,7 +767,7 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
     else
       rc = launch(ctx, "k_direct_fp64", k_direct_fp64, grid, dim3(256), A);
     if (rc) return rc;
-    if ((rc = launch(ctx, "k_reduce", k_reduce, grid1(n), dim3(256), (const double*)ctx->partial,
+    if ((rc = launch(ctx, "k_reduce", k_reduce, dim3(grid1(n).x, 3), dim3(256), (const double*)ctx->partial,
                      (int)chunks, n, cap, ctx->bs)))
       return rc;
     have_bs = 1;
@@ -1006,9 +1008,9 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
                    cap, ctx->box, ctx->a_SB)))
     return rc;
   const int n_tiles = (n + TILE - 1) / TILE;
-  // enough (tile, chunk) waves to fill 256 CUs
-  int chunks = (8192 + n_tiles - 1) / n_tiles;
-  chunks = std::max(1, std::min(chunks, std::min(256, n)));
+  // about four waves per SIMD of 256 CUs, at least 64 segments (targets) per chunk
+  int chunks = (4096 + n_tiles - 1) / n_tiles;
+  chunks = std::max(1, std::min(chunks, std::min(256, (n + 63) / 64)));
   if ((rc = grow_bytes(ctx, &ctx->a_tpart, &ctx->a_tpart_bytes, sizeof(double) * 3 * cap * chunks)) ||
       (rc = grow_bytes(ctx, &ctx->a_spart, &ctx->a_spart_bytes, sizeof(double) * 6 * cap * chunks)))
     return rc;
@@ -1028,9 +1030,13 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
   if ((rc = launch(ctx, "ka_bs_targets", ka_bs_targets, grid, dim3(256), A))) return rc;
   A.part = ctx->a_spart;
   if ((rc = launch(ctx, "ka_bs_sources", ka_bs_sources, grid, dim3(256), A))) return rc;
+  if ((rc = launch(ctx, "ka_bs_reduce", ka_bs_reduce, dim3(grid1(n).x, 9), dim3(256),
+                   (const double*)ctx->a_tpart, (const double*)ctx->a_spart, chunks, n, cap,
+                   ctx->a_gnb)))
+    return rc;
   return launch(ctx, "ka_bs_finish", ka_bs_finish, grid1(n), dim3(256), (const double*)ctx->a_gl,
-                (const double*)ctx->a_tpart, (const double*)ctx->a_spart, chunks,
-                (const int*)ctx->rev, n, cap, ctx->ph.kappa / 4 / 3.14159265358979323846, G);
+                (const double*)ctx->a_gnb, (const int*)ctx->rev, n, cap,
+                ctx->ph.kappa / 4 / 3.14159265358979323846, G);
 }
 
 // On entry G = cotangent of the step's end positions; on return the cotangent
