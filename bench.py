@@ -275,21 +275,27 @@ def main():
         rng_t = np.random.default_rng(2002)
         samp = np.sort(rng_t.choice(tg.active, size=min(4096, n), replace=False))
         tree = []
-        for prec, pname in ((T.PREC_FP64, "fp64"), (T.PREC_MIXED, "mixed")):
+        # the reference's multipole form (octree.c:303-328) in FP64 and with FP32
+        # multipoles, and the true dipole expansion for comparison
+        for prec, pname, mp in ((T.PREC_FP64, "fp64", T.MP_REFERENCE), (T.PREC_MIXED, "mixed", T.MP_REFERENCE),
+                                (T.PREC_FP64, "fp64", T.MP_DIPOLE)):
             for th in (0.3, 0.5, 0.7):
                 x = tg.copy()
                 ctx.reset_stats()
                 ctx.tree_stats()
                 ctx.velocities(x, 0.0, T.default_params(precision=prec, use_BH=1, BH_resolution=th,
-                                                        **common), profile=True)
+                                                        multipole=mp, **common), profile=True)
                 ms = ctx.last_device_ms()
                 cells, pairs, mps = ctx.tree_stats()
                 kname = "k_octree_fp64" if prec == T.PREC_FP64 else "k_octree_mixed"
-                tree.append({"theta": th, "precision": pname, "eval_ms": round(ms, 2),
+                tree.append({"theta": th, "precision": pname,
+                             "multipole": "reference" if mp == T.MP_REFERENCE else "dipole",
+                             "eval_ms": round(ms, 2),
                              "traversal_ms": round(ctx.kernel_stats(kname)[1], 2),
                              "cells": int(cells), "leaf_pairs": pairs, "multipoles": mps,
                              "err_vels_vs_fp64_direct_sample": err_stats(x.vels[samp], b.vels[samp])})
-                log(f"tree {pname} theta {th}: {ms:.1f} ms, err {tree[-1]['err_vels_vs_fp64_direct_sample']}")
+                log(f"tree {pname} {tree[-1]['multipole']} theta {th}: {ms:.1f} ms, "
+                    f"err {tree[-1]['err_vels_vs_fp64_direct_sample']}")
         out["octree"] = tree
 
     # ---- ensemble: independent tangles, one launch per stage vs a loop -------
