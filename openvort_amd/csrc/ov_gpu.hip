@@ -843,6 +843,34 @@ int finish(ov_ctx* ctx, int* flag_out) {
   return OV_OK;
 }
 
+// Capture the launches and device copies `body` issues on the context stream
+// as a graph and instantiate it (no memsets and no allocations may happen in
+// body: callers run one step eagerly first, which sizes every buffer).
+template <typename F>
+int capture_graph(ov_ctx* ctx, F body, hipGraphExec_t* exec) {
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = body();
+  hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(ctx, OV_E_DEVICE, "graph capture: %s", hipGetErrorString(ec));
+  ec = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ec != hipSuccess) return fail(ctx, OV_E_DEVICE, "graph instantiate: %s", hipGetErrorString(ec));
+  return OV_OK;
+}
+
+int launch_graph(ov_ctx* ctx, hipGraphExec_t exec, int times) {
+  for (int i = 0; i < times; ++i) {
+    hipError_t ec = hipGraphLaunch(exec, ctx->stream);
+    if (ec != hipSuccess) return fail(ctx, OV_E_DEVICE, "graph launch: %s", hipGetErrorString(ec));
+  }
+  return OV_OK;
+}
+
 // ---- ensembles ----------------------------------------------------------------
 
 int grow_int(ov_ctx* ctx, int** p, long* cap, long need) {
@@ -1278,18 +1306,31 @@ static int rk4_common(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int
     if (!t->vels) return fail(ctx, OV_E_INVALID, "vels required as k1");
     if ((rc = upload_vec3(ctx, t->vels, ctx->K[0]))) return rc;
   }
-  bool produced_t_state = false;
   if ((rc = mark_begin(ctx))) return rc;
-  for (int i = 0; i < nsteps; ++i) {
-    bool use_k1 = caller_k1 && i == 0;
-    if ((rc = rk4_device(ctx, t0 + i * dt, dt, p->rk_variant, use_k1))) return rc;
-    produced_t_state = !use_k1;
-    if (flags & OV_ENFORCE_BOUNDARIES) {
-      if ((rc = launch(ctx, "k_wrap", k_wrap, grid1(ctx->n), dim3(256), ctx->X0, ctx->n, ctx->cap,
-                       ctx->box, ctx->wall_periodic_mask)))
-        return rc;
-    }
+  auto step = [&](int i) {
+    const bool use_k1 = caller_k1 && i == 0;
+    int r = rk4_device(ctx, t0 + i * dt, dt, p->rk_variant, use_k1);
+    if (!r && (flags & OV_ENFORCE_BOUNDARIES))
+      r = launch(ctx, "k_wrap", k_wrap, grid1(ctx->n), dim3(256), ctx->X0, ctx->n, ctx->cap, ctx->box,
+                 ctx->wall_periodic_mask);
+    return r;
+  };
+  // The first step runs eagerly (it sizes every buffer); the direct-sum steps
+  // after it are one captured step replayed as a graph, which removes the
+  // per-launch overhead that dominates small tangles.  Profiling and the
+  // octree (host-read cell counts) keep eager launches.
+  const bool graphs = !ctx->profile && !ctx->use_tree;
+  int i = 0;
+  for (; i < nsteps && (i == 0 || !graphs); ++i)
+    if ((rc = step(i))) return rc;
+  if (i < nsteps) {
+    hipGraphExec_t exec = nullptr;
+    if ((rc = capture_graph(ctx, [&]() { return step(i); }, &exec))) return rc;
+    rc = launch_graph(ctx, exec, nsteps - i);
+    hipGraphExecDestroy(exec);
+    if (rc) return rc;
   }
+  const bool produced_t_state = !(caller_k1 && nsteps == 1);
   int flag = 0;
   if ((rc = finish(ctx, &flag))) return rc;
   if ((rc = download_vec3(ctx, ctx->X0, t->vnodes))) return rc;
@@ -1607,25 +1648,62 @@ int ov_gpu_rk4_run_vjp(ov_ctx* ctx, const ov_tangle_state* t, double t0, double 
                             ctx->box, ctx->wall_periodic_mask)
                    : OV_OK;
   };
+  // segment bodies: `count` steps from X0 (keeping each step's stage data when
+  // differentiating), and the VJPs of the kept steps in reverse order
+  auto forward = [&](int s0, int count, bool keep) {
+    int r = OV_OK;
+    for (int j = 0; j < count && !r; ++j) {
+      r = rk4_device(ctx, t0 + (s0 + j) * dt, dt, p->rk_variant, false,
+                     keep ? ctx->a_store + (long)j * 8 * ps : nullptr);
+      if (!r) r = wrap();
+    }
+    return r;
+  };
+  auto backward = [&](int count) {
+    int r = OV_OK;
+    for (int j = count - 1; j >= 0 && !r; --j)
+      r = step_vjp_dev(ctx, ctx->a_store + (long)j * 8 * ps, dt, p->rk_variant, ctx->a_G);
+    return r;
+  };
+  // full segments after the first eager one are graphs (no profiling)
+  const bool graphs = !ctx->profile;
+  hipGraphExec_t g_fwd = nullptr, g_seg = nullptr;
+  auto cleanup = [&](int r) {
+    if (g_fwd) hipGraphExecDestroy(g_fwd);
+    if (g_seg) hipGraphExecDestroy(g_seg);
+    return r;
+  };
   // forward sweep keeping the start of every segment (one segment: the input)
   if ((rc = copy_planes(ctx, ctx->a_ck, ctx->X0))) return rc;
-  for (int s = 0; s < (n_seg - 1) * k; ++s) {
-    if ((rc = rk4_device(ctx, t0 + s * dt, dt, p->rk_variant, false)) || (rc = wrap())) return rc;
-    if ((s + 1) % k == 0 && (rc = copy_planes(ctx, ctx->a_ck + ((s + 1) / k) * ps, ctx->X0)))
-      return rc;
+  for (int seg = 0; seg + 1 < n_seg; ++seg) {
+    if (graphs && seg > 0) {
+      if (!g_fwd && (rc = capture_graph(ctx, [&]() { return forward(seg * k, k, false); }, &g_fwd)))
+        return cleanup(rc);
+      if ((rc = launch_graph(ctx, g_fwd, 1))) return cleanup(rc);
+    } else if ((rc = forward(seg * k, k, false))) {
+      return cleanup(rc);
+    }
+    if ((rc = copy_planes(ctx, ctx->a_ck + (seg + 1) * ps, ctx->X0))) return cleanup(rc);
   }
   // reverse sweep: recompute a segment keeping every step, then its VJPs
+  bool sized = false;
   for (int seg = n_seg - 1; seg >= 0; --seg) {
-    const int s0 = seg * k, s1 = std::min(nsteps, s0 + k);
-    if ((rc = copy_planes(ctx, ctx->X0, ctx->a_ck + seg * ps))) return rc;
-    for (int s = s0; s < s1; ++s)
-      if ((rc = rk4_device(ctx, t0 + s * dt, dt, p->rk_variant, false, ctx->a_store + (s - s0) * 8 * ps)) ||
-          (rc = wrap()))
-        return rc;
-    for (int s = s1 - 1; s >= s0; --s)
-      if ((rc = step_vjp_dev(ctx, ctx->a_store + (s - s0) * 8 * ps, dt, p->rk_variant, ctx->a_G)))
-        return rc;
+    const int s0 = seg * k, count = std::min(nsteps, s0 + k) - s0;
+    if ((rc = copy_planes(ctx, ctx->X0, ctx->a_ck + seg * ps))) return cleanup(rc);
+    if (graphs && sized && count == k) {
+      if (!g_seg && (rc = capture_graph(ctx, [&]() {
+                       int r = forward(s0, k, true);
+                       return r ? r : backward(k);
+                     }, &g_seg)))
+        return cleanup(rc);
+      if ((rc = launch_graph(ctx, g_seg, 1))) return cleanup(rc);
+    } else {
+      if ((rc = forward(s0, count, true)) || (rc = backward(count))) return cleanup(rc);
+      sized = true;
+    }
   }
+  cleanup(OV_OK);
+  g_fwd = g_seg = nullptr;
   if ((rc = launch(ctx, "ka_finite", ka_finite, grid1(n), dim3(256), (const double*)ctx->a_G, n,
                    ctx->cap, ctx->flag)))
     return rc;

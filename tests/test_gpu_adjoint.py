@@ -240,3 +240,18 @@ def test_kelvin_ring_full_size_gradient_and_fd_mechanism(ctx):
             assert abs(fd - ad) <= FD_TOL * abs(ad)
         del X
         torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("nsteps,every", [(7, 2), (8, 2), (5, 0)])
+def test_rk4_run_vjp_graph_replay_matches_eager(ctx, nsteps, every):
+    """Full checkpoint segments after the first are replayed HIP graphs;
+    with profiling every launch is eager.  Every replay must give the eager
+    bits (several replays per call, eager work between calls)."""
+    t, p = AC.case("periodic_holes")
+    w = AC.seeded((t.N, 3), 17)
+    eager = ctx.rk4_run_vjp(t, 0.0, 1e-6, nsteps, p, w, checkpoint_every=every,
+                            enforce_boundaries=True, profile=True)
+    for _ in range(2):
+        g = ctx.rk4_run_vjp(t, 0.0, 1e-6, nsteps, p, w, checkpoint_every=every,
+                            enforce_boundaries=True)
+        assert np.array_equal(g, eager)

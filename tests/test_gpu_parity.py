@@ -239,3 +239,23 @@ def test_mixed_skips_wrapped_adjacent_segment(ctx, oracle, rnear):
     a, b = _both(t, lambda x: oracle.update_tangle(x, 0.0, p),
                  lambda x: ctx.velocities(x, 0.0, p, geometry=True))
     _check_vel(a, b, MIXED_TOL)
+
+
+@pytest.mark.parametrize("precision", [T.PREC_FP64, T.PREC_MIXED])
+def test_rk4_run_graph_replay_matches_eager_steps(ctx, precision):
+    """ov_gpu_rk4_run replays steps 2..n as a captured HIP graph; it must give
+    the bits of n separate ov_gpu_rk4_step calls (eager launches, host round
+    trip between steps) and of a profiled (eager) run, on every call."""
+    t = T.random_rings(61, 0.05, n_rings=3, spacing=1e-3)
+    p = tangle_params(precision=precision)
+    a = t.copy()
+    for i in range(6):
+        ctx.rk4_step(a, i * 2e-6, 2e-6, p)
+    b = t.copy()
+    ctx.rk4_run(b, 0.0, 2e-6, 6, p, profile=True)
+    assert np.array_equal(a.vnodes, b.vnodes)
+    for _ in range(2):
+        c = t.copy()
+        ctx.rk4_run(c, 0.0, 2e-6, 6, p)
+        assert np.array_equal(c.vnodes, a.vnodes)
+        assert np.array_equal(c.vels, b.vels)
