@@ -197,6 +197,22 @@ int ov_gpu_rk4_run_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t0, do
                        int nsteps, const ov_params *params, int flags, int checkpoint_every,
                        const ov_vec3d *cot_final, ov_vec3d *grad_initial);
 
+/* Reconnection candidates of reconnect (vortex_dynamics.c:140-315): every
+ * pair k < l (caller indices) of free points whose minimum-image distance is
+ * at most rec_dist and whose tangents (as the tangle holds them) both have
+ * length >= 0.5 and an angle cosine <= cos(rec_angle) — the tests of the
+ * reference's pair loop that do not change during a reconnection pass
+ * (vortex_dynamics.c:236-265), with its arithmetic.  Pairs are written as
+ * pairs[2i] = k, pairs[2i+1] = l in increasing (k, l) order, at most
+ * max_pairs of them; *n_pairs is the total (call again with a larger buffer
+ * when it exceeds max_pairs).  The host keeps the rest of the pass: visiting
+ * the candidates in order with the neighbour test, the "recalculate" flags,
+ * the velocity test and do_reconnection gives the reference's pass.  Points
+ * must lie within half a box length of a periodic box (enforce_boundaries
+ * keeps them inside). */
+int ov_gpu_reconnection_candidates(ov_ctx *ctx, const ov_tangle_state *tangle, double rec_dist,
+                                   double rec_angle, int *pairs, int max_pairs, int *n_pairs);
+
 /* Per-kernel timings gathered while OV_PROFILE was set; kernel names are the
  * device function names.  Returns the number of launches, 0 if none. */
 long ov_ctx_kernel_stats(ov_ctx *ctx, const char *kernel, double *total_ms);

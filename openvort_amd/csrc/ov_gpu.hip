@@ -22,6 +22,7 @@
 #include "ov_octree.hip"
 #include "ov_batch.hip"
 #include "ov_adjoint.hip"
+#include "ov_reconnect.hip"
 
 #include <hipcub/hipcub.hpp>
 
@@ -130,6 +131,13 @@ struct ov_ctx {
   long a_cap = 0;
   double *a_tpart = nullptr, *a_spart = nullptr, *a_ck = nullptr, *a_store = nullptr;
   size_t a_tpart_bytes = 0, a_spart_bytes = 0, a_ck_bytes = 0, a_store_bytes = 0;
+
+  // reconnection candidates
+  uint64_t *r_keys = nullptr, *r_keys_sorted = nullptr;
+  int *r_vals = nullptr, *r_order = nullptr, *r_act = nullptr, *r_count = nullptr;
+  long r_cap = 0;
+  unsigned long long *r_pairs = nullptr, *r_pairs_sorted = nullptr;
+  long r_pair_cap = 0;
 };
 
 namespace {
@@ -1121,6 +1129,37 @@ int download_grad(ov_ctx* ctx, const ov_tangle_state* t, const double* G, const 
   return download_vec3(ctx, G, grad);
 }
 
+
+// ---- reconnection candidates ---------------------------------------------------
+
+int rec_buffers(ov_ctx* ctx, long n) {
+  int rc;
+  if (ctx->r_cap < ctx->cap) {
+    const long c = ctx->cap;
+    if ((rc = dalloc(ctx, (void**)&ctx->r_keys, sizeof(uint64_t) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->r_keys_sorted, sizeof(uint64_t) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->r_vals, sizeof(int) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->r_order, sizeof(int) * c)) ||
+        (rc = dalloc(ctx, (void**)&ctx->r_act, sizeof(int) * c)))
+      return rc;
+    ctx->r_cap = c;
+  }
+  if (!ctx->r_count && (rc = dalloc(ctx, (void**)&ctx->r_count, sizeof(int)))) return rc;
+  (void)n;
+  return OV_OK;
+}
+
+int rec_pair_buffers(ov_ctx* ctx, long need) {
+  if (need <= ctx->r_pair_cap) return OV_OK;
+  const long c = std::max<long>(need, 2 * ctx->r_pair_cap);
+  int rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->r_pairs, sizeof(unsigned long long) * c)) ||
+      (rc = dalloc(ctx, (void**)&ctx->r_pairs_sorted, sizeof(unsigned long long) * c)))
+    return rc;
+  ctx->r_pair_cap = c;
+  return OV_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1193,6 +1232,10 @@ void ov_ctx_destroy(ov_ctx* ctx) {
                 ctx->a_G,     ctx->a_Gtmp,  ctx->a_gK[0], ctx->a_gK[1], ctx->a_gK[2],
                 ctx->a_gK[3], ctx->a_tpart, ctx->a_spart, ctx->a_ck,    ctx->a_store};
   for (void* b : ab)
+    if (b) hipFree(b);
+  void* rb[] = {ctx->r_keys, ctx->r_keys_sorted, ctx->r_vals,  ctx->r_order,
+                ctx->r_act,  ctx->r_count,       ctx->r_pairs, ctx->r_pairs_sorted};
+  for (void* b : rb)
     if (b) hipFree(b);
   if (ctx->hst) hipHostFree(ctx->hst);
   for (auto& p : ctx->pending) {
@@ -1713,6 +1756,103 @@ int ov_gpu_rk4_run_vjp(ov_ctx* ctx, const ov_tangle_state* t, double t0, double 
   if (flag)
     return fail(ctx, OV_E_NONFINITE, "non-finite %s",
                 (flag & 4) ? "gradient" : (flag & 2) ? "position" : "velocity");
+  return OV_OK;
+}
+
+
+int ov_gpu_reconnection_candidates(ov_ctx* ctx, const ov_tangle_state* t, double rec_dist,
+                                   double rec_angle, int* pairs, int max_pairs, int* n_pairs) {
+  if (!ctx) return OV_E_INVALID;
+  if (!n_pairs || (max_pairs > 0 && !pairs)) return fail(ctx, OV_E_INVALID, "null output");
+  if (!(rec_dist > 0) || !std::isfinite(rec_dist) || !std::isfinite(rec_angle))
+    return fail(ctx, OV_E_INVALID, "bad rec_dist %g or rec_angle %g", rec_dist, rec_angle);
+  *n_pairs = 0;
+  HIPCHK(hipSetDevice(ctx->device));
+  ov_params p;
+  ov_params_default(&p);
+  int rc = prepare(ctx, t, &p);
+  if (rc) return rc;
+  const int n = ctx->n;
+  if (n < 2) return OV_OK;
+  if (!t->tangents) return fail(ctx, OV_E_INVALID, "tangents required");
+  ctx->profile = false;
+  const long cap = ctx->cap;
+  if ((rc = rec_buffers(ctx, n)) || (rc = upload_vec3(ctx, t->vnodes, ctx->X0)) ||
+      (rc = upload_vec3(ctx, t->tangents, ctx->T)))
+    return rc;
+  HIPCHK(hipMemcpyAsync(ctx->r_act, ctx->act.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  // cells of edge >= rec_dist; a periodic axis is cut into whole cells
+  CellGrid g{};
+  for (int ax = 0; ax < 3; ++ax) {
+    g.lo[ax] = ctx->box.lo[ax];
+    if (ctx->box.periodic[ax] && ctx->box.L[ax] > 0) {
+      const double cells = std::floor(ctx->box.L[ax] / rec_dist);
+      g.dims[ax] = (int)std::max(1.0, std::min(cells, 1048576.0));
+      g.h[ax] = ctx->box.L[ax] / g.dims[ax];
+    } else {
+      g.dims[ax] = 0;
+      g.h[ax] = rec_dist;
+    }
+  }
+  if ((rc = mark_begin(ctx))) return rc;
+  if ((rc = launch(ctx, "kr_keys", kr_keys, grid1(n), dim3(256), (const double*)ctx->X0, n, cap, g,
+                   ctx->r_keys, ctx->r_vals)))
+    return rc;
+  size_t need = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, ctx->r_keys, ctx->r_keys_sorted,
+                                            ctx->r_vals, ctx->r_order, n, 0, 63, ctx->stream));
+  if ((rc = cub_temp(ctx, need))) return rc;
+  size_t tb = ctx->t_cub_bytes;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->t_cub, tb, ctx->r_keys, ctx->r_keys_sorted,
+                                            ctx->r_vals, ctx->r_order, n, 0, 63, ctx->stream));
+  RecArgs A;
+  A.P = ctx->X0;
+  A.T = ctx->T;
+  A.act = ctx->r_act;
+  A.keys = ctx->r_keys_sorted;
+  A.order = ctx->r_order;
+  A.cap = cap;
+  A.n = n;
+  A.g = g;
+  A.box = ctx->box;
+  A.rec_dist = rec_dist;
+  A.cos_angle = std::cos(rec_angle);
+  A.count = ctx->r_count;
+  if ((rc = rec_pair_buffers(ctx, std::max<long>(4L * n, 1024)))) return rc;
+  int count = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    A.pairs = ctx->r_pairs;
+    A.max_pairs = (int)std::min<long>(ctx->r_pair_cap, 0x7fffffffL);
+    HIPCHK(hipMemsetAsync(ctx->r_count, 0, sizeof(int), ctx->stream));
+    if ((rc = launch(ctx, "kr_pairs", kr_pairs, grid1(n), dim3(256), A))) return rc;
+    HIPCHK(hipMemcpyAsync(&count, ctx->r_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (count < 0) return fail(ctx, OV_E_NOMEM, "more than 2^31 candidate pairs");
+    if (count <= A.max_pairs) break;
+    if ((rc = rec_pair_buffers(ctx, count))) return rc;   // grow and search again
+  }
+  if (count > 1) {
+    need = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, need, ctx->r_pairs, ctx->r_pairs_sorted, count,
+                                             0, 64, ctx->stream));
+    if ((rc = cub_temp(ctx, need))) return rc;
+    tb = ctx->t_cub_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(ctx->t_cub, tb, ctx->r_pairs, ctx->r_pairs_sorted, count,
+                                             0, 64, ctx->stream));
+  }
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  *n_pairs = count;
+  const long out = std::min<long>(count, std::max(max_pairs, 0));
+  if (out > 0) {
+    std::vector<unsigned long long> h(out);
+    HIPCHK(hipMemcpy(h.data(), count > 1 ? ctx->r_pairs_sorted : ctx->r_pairs,
+                     sizeof(unsigned long long) * out, hipMemcpyDeviceToHost));
+    for (long i = 0; i < out; ++i) {
+      pairs[2 * i] = (int)(h[i] >> 32);
+      pairs[2 * i + 1] = (int)(h[i] & 0xffffffffu);
+    }
+  }
   return OV_OK;
 }
 

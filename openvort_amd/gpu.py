@@ -38,6 +38,7 @@ SYMBOLS = (
     "ov_ctx_batch_tuning",
     "ov_gpu_velocity_vjp",
     "ov_gpu_rk4_run_vjp",
+    "ov_gpu_reconnection_candidates",
 )
 
 _lib = None
@@ -98,6 +99,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.ov_gpu_rk4_run_vjp.argtypes = [C.c_void_p, P(TangleState), C.c_double, C.c_double, C.c_int,
                                        P(Params), C.c_int, C.c_int, V3, V3]
     lib.ov_gpu_rk4_run_vjp.restype = C.c_int
+    lib.ov_gpu_reconnection_candidates.argtypes = [C.c_void_p, P(TangleState), C.c_double, C.c_double,
+                                                   C.c_void_p, C.c_int, P(C.c_int)]
+    lib.ov_gpu_reconnection_candidates.restype = C.c_int
     _lib = lib
     return lib
 
@@ -196,6 +200,21 @@ class Context:
                                          checkpoint_every, w.ctypes.data, g.ctypes.data)
         self._check(rc)
         return g
+
+    def reconnection_candidates(self, t: Tangle, rec_dist: float, rec_angle: float):
+        """(n, 2) int32 array of candidate pairs (k < l, caller indices) in
+        (k, l) order: the pass-invariant tests of the reference's reconnect."""
+        n = C.c_int(0)
+        cap = max(4 * t.N, 1024)
+        while True:
+            buf = np.zeros((cap, 2), dtype=np.int32)
+            ts = t.cstruct()
+            rc = self.lib.ov_gpu_reconnection_candidates(self.ptr, C.byref(ts), rec_dist, rec_angle,
+                                                         buf.ctypes.data, cap, C.byref(n))
+            self._check(rc)
+            if n.value <= cap:
+                return buf[:n.value].copy()
+            cap = n.value
 
     def batch_tuning(self):
         u, b, ms = C.c_int(), C.c_int(), C.c_double()

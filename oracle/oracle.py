@@ -49,6 +49,9 @@ class Oracle:
             lib.ovo_octree_stats.restype = None
             lib.ovo_set_threads.argtypes = [C.c_int]
             lib.ovo_set_threads.restype = None
+            lib.ovo_reconnection_candidates.argtypes = [P(TangleState), C.c_double, C.c_double,
+                                                        P(C.c_int), C.c_long]
+            lib.ovo_reconnection_candidates.restype = C.c_long
             Oracle._lib = lib
         self.lib = Oracle._lib
         self.lib.ovo_set_threads(threads)
@@ -104,6 +107,15 @@ class Oracle:
                                    out.ctypes.data_as(P(C.c_double)))
         return out
 
+    def reconnection_candidates(self, t: Tangle, rec_dist: float, rec_angle: float):
+        """(n, 2) int32 pairs k < l passing the pass-invariant reconnection tests."""
+        ts = t.cstruct()
+        n = self.lib.ovo_reconnection_candidates(C.byref(ts), rec_dist, rec_angle, None, 0)
+        out = np.zeros((max(n, 1), 2), dtype=np.int32)
+        self.lib.ovo_reconnection_candidates(C.byref(ts), rec_dist, rec_angle,
+                                             out.ctypes.data_as(P(C.c_int)), n)
+        return out[:n]
+
     def octree_stats(self):
         a, b, c = C.c_long(), C.c_long(), C.c_long()
         self.lib.ovo_octree_stats(C.byref(a), C.byref(b), C.byref(c))
@@ -135,6 +147,10 @@ class Reference:
             lib.ovref_rk4_step.restype = None
             lib.ovref_enforce_boundaries.argtypes = [P(TangleState)]
             lib.ovref_enforce_boundaries.restype = None
+            lib.ovref_reconnect.argtypes = [P(TangleState), C.c_double, C.c_double, C.c_double]
+            lib.ovref_reconnect.restype = C.c_int
+            lib.ovref_reconnect_candidates.argtypes = [P(TangleState), C.c_double, P(C.c_int), C.c_long]
+            lib.ovref_reconnect_candidates.restype = C.c_int
             Reference._lib = lib
         self.lib = Reference._lib
         self.threads = threads
@@ -180,3 +196,17 @@ class Reference:
     def enforce_boundaries(self, t: Tangle):
         ts = t.cstruct()
         self.lib.ovref_enforce_boundaries(C.byref(ts))
+
+    def reconnect(self, t: Tangle, time: float, p: Params, rec_dist: float, rec_angle: float) -> int:
+        """The reference's reconnect pass (vortex_dynamics.c:140-315)."""
+        self.set_params(p)
+        ts = t.cstruct()
+        return self.lib.ovref_reconnect(C.byref(ts), time, rec_dist, rec_angle)
+
+    def reconnect_candidates(self, t: Tangle, time: float, p: Params, pairs) -> int:
+        """The same pass driven by a sorted candidate list (ref_harness.c)."""
+        self.set_params(p)
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        ts = t.cstruct()
+        return self.lib.ovref_reconnect_candidates(C.byref(ts), time, pairs.ctypes.data_as(P(C.c_int)),
+                                                   len(pairs))

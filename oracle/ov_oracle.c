@@ -670,3 +670,35 @@ void ovo_enforce_boundaries(ov_tangle_state *t) {
     }
   }
 }
+
+/* Pass-invariant tests of the reconnection pair loop (vortex_dynamics.c:
+ * 221-265), every pair k < l in the loop's order: both points free, the
+ * minimum-image distance |v_k - pwrap(v_k, v_l)| <= rec_dist (vec3_dist,
+ * vec3_maths.c:347-351), |t_k|, |t_l| >= 0.5 and the tangents' normalised
+ * dot (vec3_ndot, vec3_maths.c:295-301) <= cos(rec_angle). */
+long ovo_reconnection_candidates(const ov_tangle_state *t, double rec_dist, double rec_angle,
+                                 int *pairs, long max_pairs) {
+  const double ca = cos(rec_angle);
+  long count = 0;
+  for (int k = 0; k < t->N; ++k) {
+    if (t->status[k].status != OV_FREE) continue;
+    const double *a = t->vnodes[k].p, *ta = t->tangents[k].p;
+    for (int l = k + 1; l < t->N; ++l) {
+      if (t->status[l].status != OV_FREE) continue;
+      double w[3], d[3];
+      pwrap(a, t->vnodes[l].p, &t->box, w);
+      for (int i = 0; i < 3; ++i) d[i] = a[i] - w[i];
+      if (norm3(d) > rec_dist) continue;
+      const double *tb = t->tangents[l].p;
+      const double na = norm3(ta), nb = norm3(tb);
+      if (na < 0.5 || nb < 0.5) continue;
+      if (dot3(ta, tb) / na / nb > ca) continue;
+      if (count < max_pairs) {
+        pairs[2 * count] = k;
+        pairs[2 * count + 1] = l;
+      }
+      ++count;
+    }
+  }
+  return count;
+}

@@ -46,6 +46,12 @@ void ovo_segment_field(const double a[3], const double b[3], const double r[3],
  * non-empty cells, evaluated leaf segment-point pairs, multipoles. */
 void ovo_octree_stats(long *n_cells, long *n_leaf_pairs, long *n_multipoles);
 
+/* Pass-invariant reconnection tests over all pairs k < l (vortex_dynamics.c:
+ * 221-265): writes up to max_pairs (k, l) pairs in loop order, returns the
+ * total count. */
+long ovo_reconnection_candidates(const ov_tangle_state *t, double rec_dist, double rec_angle,
+                                 int *pairs, long max_pairs);
+
 /* Threads used by the OpenMP loops (0: runtime default). */
 void ovo_set_threads(int n);
 

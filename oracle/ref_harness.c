@@ -116,3 +116,62 @@ void ovref_rk4_step(ov_tangle_state *t, double time, double dt) {
 void ovref_enforce_boundaries(ov_tangle_state *t) {
   enforce_boundaries((struct tangle_state *)t);
 }
+
+/* The reference's reconnect (vortex_dynamics.c:140-315) with the reconnection
+ * globals it reads; returns the number of reconnections. */
+int ovref_reconnect(ov_tangle_state *t, double time, double rdist, double rangle) {
+  return reconnect((struct tangle_state *)t, time, rdist, rangle);
+}
+
+/* The vortex-vortex part of the same pass driven by a candidate list (pairs
+ * k < l in increasing (k, l) order, the pair-loop tests that do not change
+ * during the pass already applied): the tests that do change — the
+ * "recalculate" flags, the neighbour test, the approach test on the two
+ * points' velocities — and the rewiring are the reference's own calls, in the
+ * order of its loop (vortex_dynamics.c:214-312).  For boxes without walls it
+ * leaves the tangle as ovref_reconnect does; returns the reconnections. */
+int do_reconnection(struct tangle_state *tangle, size_t k, size_t l);   /* vortex_dynamics.c:317 */
+
+int ovref_reconnect_candidates(ov_tangle_state *t, double time, const int *pairs, long n_pairs) {
+  struct tangle_state *ts = (struct tangle_state *)t;
+  int recs = 0;
+  for (int q = 0; q < ts->N; ++q) ts->recalculate[q] = 0;
+  long i = 0;
+  while (i < n_pairs) {
+    const int k = pairs[2 * i];
+    long next_k = i;
+    while (next_k < n_pairs && pairs[2 * next_k] == k) ++next_k;   /* this k's run */
+    if (!ts->recalculate[k]) {
+      for (long j = i; j < next_k; ++j) {
+        const int l = pairs[2 * j + 1];
+        if (ts->connections[k].forward == l || ts->connections[k].reverse == l ||
+            ts->recalculate[l])
+          continue;
+        struct segment seg = seg_pwrap(&ts->vnodes[k], &ts->vnodes[l], &ts->box);
+        update_velocity(ts, k, time, NULL);
+        update_velocity(ts, l, time, NULL);
+        struct vec3d dx, dv;
+        vec3_sub(&dx, &seg.r1, &seg.r2);
+        vec3_sub(&dv, &ts->vels[k], &ts->vels[l]);
+        if (vec3_dot(&dx, &dv) > 0) continue;
+        const int knext = ts->connections[k].forward, kprev = ts->connections[k].reverse;
+        const int lnext = ts->connections[l].forward, lprev = ts->connections[l].reverse;
+        if (!do_reconnection(ts, k, l)) continue;
+        ts->recalculate[k]++;
+        ts->recalculate[ts->connections[k].forward]++;
+        ts->recalculate[ts->connections[k].reverse]++;
+        ts->recalculate[knext]++;
+        ts->recalculate[kprev]++;
+        ts->recalculate[l]++;
+        ts->recalculate[ts->connections[l].forward]++;
+        ts->recalculate[ts->connections[l].reverse]++;
+        ts->recalculate[lnext]++;
+        ts->recalculate[lprev]++;
+        ++recs;
+        break;
+      }
+    }
+    i = next_k;
+  }
+  return recs;
+}
