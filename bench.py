@@ -22,7 +22,9 @@ same positions and compared with the FP64 direct sum on 4096 seeded targets.
 An ensemble of 256 independent periodic tangles (box 0.05 cm, 500-20000
 points each) is stepped in one launch per stage and, for comparison, one
 tangle at a time.  The adjoint is timed on the Kelvin-ring loss (4096
-points, 200 RK4 steps) and checked against central differences.
+points, 200 RK4 steps) and checked against central differences.  The
+reconnection candidate search (the reference reconnect's pair loop) is timed
+on the workload's positions.
 """
 from __future__ import annotations
 
@@ -179,6 +181,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=2)
     ap.add_argument("--adjoint", type=int, default=1,
                     help="Kelvin-ring gradient (4096 points, 200 steps, checkpoints every 20; 0: skip)")
+    ap.add_argument("--reconnect", type=int, default=1,
+                    help="reconnection candidate search on the workload (0: skip)")
     ap.add_argument("--tree", type=int, default=1,
                     help="octree evaluations at theta 0.3/0.5/0.7 against the FP64 direct sum (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -300,6 +304,23 @@ def main():
                 log(f"tree {pname} {tree[-1]['multipole']} theta {th}: {ms:.1f} ms, "
                     f"err {tree[-1]['err_vels_vs_fp64_direct_sample']}")
         out["octree"] = tree
+
+    # ---- reconnection candidates (reference reconnect's pair loop) ----------
+    if args.reconnect > 0:
+        x = tg.copy()
+        ctx.update_tangents_normals(x)
+        rec = []
+        for rd in (1e-3, 2e-4):
+            ctx.reconnection_candidates(x, rd, 0.087)          # sizes the buffers
+            w0 = time.time()
+            cand = ctx.reconnection_candidates(x, rd, 0.087)
+            wall_ms = (time.time() - w0) * 1e3
+            rec.append({"rec_dist_cm": rd, "rec_angle": 0.087, "candidates": int(len(cand)),
+                        "device_ms": round(ctx.last_device_ms(), 2), "wall_ms": round(wall_ms, 2),
+                        "reference_pair_tests": n * (n - 1) // 2})
+            log(f"reconnection candidates rec_dist {rd}: {len(cand)} pairs, "
+                f"{rec[-1]['device_ms']} ms device, {wall_ms:.1f} ms wall")
+        out["reconnection_candidates"] = rec
 
     # ---- ensemble: independent tangles, one launch per stage vs a loop -------
     if args.ensemble > 0:

@@ -169,8 +169,9 @@ int ov_gpu_rk4_run(ov_ctx *ctx, ov_tangle_state *tangle, double t0, double dt,
  * No counterpart in the reference (it holds one tangle per process). */
 int ov_gpu_rk4_run_batch(ov_ctx *ctx, ov_tangle_state *const *tangles, int ntangles,
                          double t0, double dt, int nsteps, const ov_params *params, int flags);
-/* Unit size (work items per unit), persistent grid (blocks of 4 waves) and
- * the tuning time of the last batch layout. */
+/* Unit size (work items per unit), persistent grid (blocks of 4 waves; 0:
+ * one unit per wavefront, no queue) and the tuning time of the last batch
+ * layout. */
 void ov_ctx_batch_tuning(const ov_ctx *ctx, int *unit, int *blocks, double *tune_ms);
 
 /* Adjoint of update_tangle + update_velocities (tangle.c:583-702, geometry

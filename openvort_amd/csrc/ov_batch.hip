@@ -292,15 +292,22 @@ struct BatchArgs {
   double* upart;         // [unit][64 lanes][3]
 };
 
-// Persistent direct sum over all tangles' work units.
-template <bool MIXED>
+// Direct sum over all tangles' work units.  PERSIST: a fixed grid of
+// wavefronts takes units from an atomic queue until none are left;
+// otherwise one unit per wavefront (unit = block * 4 + wave; the grid is a
+// bound on the unit count and the surplus waves exit at once).
+template <bool MIXED, bool PERSIST>
 __device__ __forceinline__ void batch_body(const BatchArgs& A) {
   const int lane = threadIdx.x & 63;
   const int total = A.ucounts[0];
   for (;;) {
     int u = 0;
-    if (lane == 0) u = atomicAdd(A.queue, 1);
-    u = __builtin_amdgcn_readfirstlane(u);
+    if (PERSIST) {
+      if (lane == 0) u = atomicAdd(A.queue, 1);
+      u = __builtin_amdgcn_readfirstlane(u);
+    } else {
+      u = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    }
     if (u >= total) break;
     // target tile of unit u: the last T with offset(T) <= u
     int lo = 0, hi = A.n_tgt - 1;
@@ -330,11 +337,14 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A) {
     o[0] = acc[0] * t.kappa4pi;
     o[1] = acc[1] * t.kappa4pi;
     o[2] = acc[2] * t.kappa4pi;
+    if (!PERSIST) break;
   }
 }
 
-__global__ void __launch_bounds__(256) kb_direct_mixed(BatchArgs A) { batch_body<true>(A); }
-__global__ void __launch_bounds__(256) kb_direct_fp64(BatchArgs A) { batch_body<false>(A); }
+__global__ void __launch_bounds__(256) kb_direct_mixed(BatchArgs A) { batch_body<true, true>(A); }
+__global__ void __launch_bounds__(256) kb_direct_fp64(BatchArgs A) { batch_body<false, true>(A); }
+__global__ void __launch_bounds__(256, 4) kb_grid_mixed(BatchArgs A) { batch_body<true, false>(A); }
+__global__ void __launch_bounds__(256) kb_grid_fp64(BatchArgs A) { batch_body<false, false>(A); }
 
 // Sum of each target's unit partials in unit order.
 __global__ void __launch_bounds__(256) kb_reduce(const double* __restrict__ upart,

@@ -889,6 +889,17 @@ int grow_int(ov_ctx* ctx, int** p, long* cap, long need) {
   return rc;
 }
 
+// Upper bound of the work units of one evaluation at unit size U (a tangle
+// of n points has at most ceil(2n / 64) source tiles).
+long batch_unit_bound(const ov_ctx* ctx, int U) {
+  long units = 0;
+  for (const TangleDev& t : ctx->b_tg) {
+    const long items = (long)((2 * t.n + TILE - 1) / TILE) * t.n_img;
+    units += (long)t.n_tgt * ((items + U - 1) / U);
+  }
+  return units;
+}
+
 // One velocity evaluation of every tangle of the batch at positions X.
 int evaluate_batch(ov_ctx* ctx, const double* X, double* Tg, double* Ng, double* dxi, double* vs,
                    double* vel, float* direct_ms) {
@@ -955,10 +966,18 @@ int evaluate_batch(ov_ctx* ctx, const double* X, double* Tg, double* Ng, double*
     e1 = get_event(ctx);
     HIPCHK(hipEventRecord(e0, ctx->stream));
   }
-  if (ctx->b_mixed)
-    rc = launch(ctx, "kb_direct_mixed", kb_direct_mixed, dim3(ctx->b_blocks), dim3(256), A);
-  else
-    rc = launch(ctx, "kb_direct_fp64", kb_direct_fp64, dim3(ctx->b_blocks), dim3(256), A);
+  if (ctx->b_blocks > 0) {   // persistent grid over the unit queue
+    if (ctx->b_mixed)
+      rc = launch(ctx, "kb_direct_mixed", kb_direct_mixed, dim3(ctx->b_blocks), dim3(256), A);
+    else
+      rc = launch(ctx, "kb_direct_fp64", kb_direct_fp64, dim3(ctx->b_blocks), dim3(256), A);
+  } else {                   // one unit per wavefront
+    const unsigned g = (unsigned)((batch_unit_bound(ctx, ctx->b_U) + 3) / 4);
+    if (ctx->b_mixed)
+      rc = launch(ctx, "kb_grid_mixed", kb_grid_mixed, dim3(std::max(g, 1u)), dim3(256), A);
+    else
+      rc = launch(ctx, "kb_grid_fp64", kb_grid_fp64, dim3(std::max(g, 1u)), dim3(256), A);
+  }
   if (rc) return rc;
   if (direct_ms) {
     HIPCHK(hipEventRecord(e1, ctx->stream));
@@ -1547,7 +1566,7 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
     std::pair<int, int> pick(256, dev_cus * 4);
     double tune_ms = 0;
     for (int U : Us) {
-      for (int m : {2, 4, 8}) {
+      for (int m : {0, 2, 4, 8}) {   // 0: one unit per wavefront, else a persistent grid
         ctx->b_U = U;
         ctx->b_blocks = dev_cus * m;
         float ms = 0;
