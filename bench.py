@@ -57,6 +57,23 @@ def err_stats(x, y):
             "max_abs_over_rms": float(d.max() / np.sqrt(np.mean(n * n)))}
 
 
+# octree build passes timed per evaluation (the radix sort runs inside hipcub)
+OCT_BUILD = ("k_oct_keys", "k_gather_u64", "k_oct_leafsegs", "k_oct_count", "k_oct_emit",
+             "k_oct_moments", "k_oct_size", "k_oct_pre", "k_oct_nodes")
+
+
+def tree_hbm_bytes():
+    """HBM bytes of each octree build pass per evaluation of the 1M-point
+    workload, from the committed PMC summary (copy-calibrated FETCH_SIZE +
+    WRITE_SIZE); {} when absent."""
+    path = os.path.join(ROOT, "profiles", "octree_build_bh1m_pmc.md")
+    try:
+        text = open(path).read()
+        return json.loads(text.split("```json", 1)[1].split("```", 1)[0])["hbm_bytes_per_evaluation"]
+    except (OSError, IndexError, ValueError, KeyError):
+        return {}
+
+
 def measured_traffic(points, seed, near_radius):
     """HBM bytes per launch of k_direct_mixed from the committed PMC summary
     (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), when it covers
@@ -278,7 +295,7 @@ def main():
             f"err {out['err_vels_mixed_vs_fp64']}")
 
     # ---- Barnes-Hut octree vs the FP64 direct sum (seeded 4096-target sample)
-    if args.tree > 0 and args.fp64 > 0:
+    if args.tree > 0:
         rng_t = np.random.default_rng(2002)
         samp = np.sort(rng_t.choice(tg.active, size=min(4096, n), replace=False))
         tree = []
@@ -300,9 +317,18 @@ def main():
                              "eval_ms": round(ms, 2),
                              "traversal_ms": round(ctx.kernel_stats(kname)[1], 2),
                              "cells": int(cells), "leaf_pairs": pairs, "multipoles": mps,
-                             "err_vels_vs_fp64_direct_sample": err_stats(x.vels[samp], b.vels[samp])})
+                             "build_kernels_ms": {kn: round(ctx.kernel_stats(kn)[1], 3) for kn in OCT_BUILD}})
+                if args.fp64 > 0:
+                    tree[-1]["err_vels_vs_fp64_direct_sample"] = err_stats(x.vels[samp], b.vels[samp])
                 log(f"tree {pname} {tree[-1]['multipole']} theta {th}: {ms:.1f} ms, "
-                    f"err {tree[-1]['err_vels_vs_fp64_direct_sample']}")
+                    f"err {tree[-1].get('err_vels_vs_fp64_direct_sample')}")
+        hbm = tree_hbm_bytes()
+        if hbm:
+            build = tree[0]["build_kernels_ms"]
+            out["octree_build_hbm"] = {
+                kn: {"hbm_bytes": hbm[kn], "ms": build[kn],
+                     "GB_per_s": round(hbm[kn] / (build[kn] / 1e3) / 1e9, 1) if build.get(kn) else None}
+                for kn in OCT_BUILD if kn in hbm}
         out["octree"] = tree
 
     # ---- reconnection candidates (reference reconnect's pair loop) ----------

@@ -104,6 +104,10 @@ struct ov_ctx {
   void* t_cub = nullptr;
   size_t t_cub_bytes = 0;
   long* t_stats = nullptr;
+  int* t_big = nullptr;
+  long t_big_cap = 0;
+  double* t_big_part = nullptr;
+  size_t t_big_part_bytes = 0;
   long tree_cells = 0;
   double tree_leaf_pairs = 0, tree_multipoles = 0;
   hipEvent_t t_begin = nullptr, t_end = nullptr;
@@ -529,6 +533,14 @@ int tree_cell_buffers(ov_ctx* ctx, long need, long keep) {
   return OV_OK;
 }
 
+int grow_bytes_int(ov_ctx* ctx, int** p, long* cap, long need) {
+  if (need <= *cap) return OV_OK;
+  const long c = std::max<long>(need, 2 * *cap);
+  int rc = dalloc(ctx, (void**)p, sizeof(int) * c);
+  if (!rc) *cap = c;
+  return rc;
+}
+
 int cub_temp(ov_ctx* ctx, size_t bytes) {
   if (bytes <= ctx->t_cub_bytes) return OV_OK;
   int rc = dalloc(ctx, &ctx->t_cub, bytes);
@@ -630,9 +642,34 @@ int build_tree(ov_ctx* ctx, const double* X) {
   }
   const int ncell = le;
   ctx->tree_cells = ncell;
+  // cells of more than OCT_BIG points are listed and summed in slices
+  if ((rc = grow_bytes_int(ctx, &ctx->t_big, &ctx->t_big_cap, std::max(ncell, 1)))) return rc;
+  HIPCHK(hipMemsetAsync(ctx->t_maxd, 0, sizeof(int), ctx->stream));
   if ((rc = launch(ctx, "k_oct_moments", k_oct_moments, dim3((unsigned)((ncell + 3) / 4)), dim3(256),
-                   (const OctCell*)ctx->t_cells, ncell, (const LeafSeg*)ctx->t_segs, ctx->t_mom)))
+                   (const OctCell*)ctx->t_cells, ncell, (const LeafSeg*)ctx->t_segs, ctx->t_mom,
+                   ctx->t_big, ctx->t_maxd)))
     return rc;
+  int n_big = 0;
+  HIPCHK(hipMemcpyAsync(&n_big, ctx->t_maxd, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (n_big > 0) {
+    const size_t pb = sizeof(double) * 15 * OCT_SLICES * (size_t)n_big;
+    if (pb > ctx->t_big_part_bytes) {
+      if ((rc = dalloc(ctx, (void**)&ctx->t_big_part, pb))) return rc;
+      ctx->t_big_part_bytes = pb;
+    }
+    const dim3 gb(OCT_SLICES, (unsigned)n_big);
+    if ((rc = launch(ctx, "k_oct_big_sum1", k_oct_big_sum1, gb, dim3(256), (const OctCell*)ctx->t_cells,
+                     (const int*)ctx->t_big, (const int*)ctx->t_maxd, (const LeafSeg*)ctx->t_segs,
+                     ctx->t_big_part)) ||
+        (rc = launch(ctx, "k_oct_big_sum2", k_oct_big_sum2, gb, dim3(256), (const OctCell*)ctx->t_cells,
+                     (const int*)ctx->t_big, (const int*)ctx->t_maxd, (const LeafSeg*)ctx->t_segs,
+                     ctx->t_big_part)) ||
+        (rc = launch(ctx, "k_oct_big_fin", k_oct_big_fin, dim3((unsigned)((n_big + 63) / 64)), dim3(64),
+                     (const OctCell*)ctx->t_cells, (const int*)ctx->t_big, (const int*)ctx->t_maxd,
+                     (const double*)ctx->t_big_part, ctx->t_mom)))
+      return rc;
+  }
   for (int L = (int)levels.size() - 1; L >= 0; --L) {
     const int a = levels[L].first, b = levels[L].second;
     if ((rc = launch(ctx, "k_oct_size", k_oct_size, dim3((unsigned)((b - a + 255) / 256)), dim3(256),
@@ -1239,7 +1276,8 @@ void ov_ctx_destroy(ov_ctx* ctx) {
   void* tb[] = {ctx->t_order, ctx->t_k[0], ctx->t_k[1], ctx->t_k[2], ctx->t_ks[0], ctx->t_ks[1],
                 ctx->t_ks[2], ctx->t_ktmp, ctx->t_kout, ctx->t_vals, ctx->t_vals2, ctx->t_sorted,
                 ctx->t_maxd, ctx->t_segs, ctx->t_cells, ctx->t_nchild, ctx->t_offs, ctx->t_size,
-                ctx->t_pre, ctx->t_mom, ctx->t_nodes, ctx->t_cub, ctx->t_stats};
+                ctx->t_pre, ctx->t_mom, ctx->t_nodes, ctx->t_cub, ctx->t_stats, ctx->t_big,
+                ctx->t_big_part};
   for (void* b : tb)
     if (b) hipFree(b);
   void* bb[] = {ctx->b_dtg,  ctx->b_tid,   ctx->b_tile_tangle, ctx->b_tgt_tangle, ctx->b_tM,

@@ -278,13 +278,23 @@ __device__ inline double wave_sum(double v) {
 // wavefront per cell: lane sums over a stride of 64, then a fixed butterfly
 // (deterministic; the order differs from the reference's sequential sum by
 // rounding only).
+// Cells of more than OCT_BIG points are only listed here (big, n_big) and
+// summed by the k_oct_big_* kernels, OCT_SLICES blocks per cell.
+constexpr int OCT_BIG = 8192;
+constexpr int OCT_SLICES = 64;
+
 __global__ void __launch_bounds__(256) k_oct_moments(const OctCell* __restrict__ cells, int ncell,
                                                      const LeafSeg* __restrict__ segs,
-                                                     double* __restrict__ mom) {
+                                                     double* __restrict__ mom, int* __restrict__ big,
+                                                     int* __restrict__ n_big) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncell) return;
   const OctCell cell = cells[c];
+  if (cell.count > OCT_BIG) {
+    if (lane == 0) big[atomicAdd(n_big, 1)] = c;
+    return;
+  }
   const int e = cell.start + cell.count;
   double s[3] = {0, 0, 0};
   for (int j = cell.start + lane; j < e; j += 64)
@@ -313,6 +323,108 @@ __global__ void __launch_bounds__(256) k_oct_moments(const OctCell* __restrict__
     }
     for (int a = 0; a < 9; ++a) o[6 + a] = M[a];
   }
+}
+
+// Block-wide sum of one value per thread (256 threads): the four wave sums
+// added in wave order (deterministic).
+__device__ inline double block_sum256(double v, double* lds) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+// Big cells, pass 1: per slice the sums of the points and of the segment
+// vectors (part[cell][slice][6]); slice x covers [start + x count / S,
+// start + (x + 1) count / S).
+__global__ void __launch_bounds__(256) k_oct_big_sum1(const OctCell* __restrict__ cells,
+                                                      const int* __restrict__ big,
+                                                      const int* __restrict__ n_big,
+                                                      const LeafSeg* __restrict__ segs,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[4];
+  const int y = blockIdx.y;
+  if (y >= *n_big) return;
+  const OctCell cell = cells[big[y]];
+  const long b = cell.start + (long)cell.count * blockIdx.x / OCT_SLICES;
+  const long e = cell.start + (long)cell.count * (blockIdx.x + 1) / OCT_SLICES;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (long j = b + threadIdx.x; j < e; j += 256) {
+    const LeafSeg q = segs[j];
+    for (int a = 0; a < 3; ++a) {
+      s[a] += q.a[a];
+      s[3 + a] += q.b[a] - q.a[a];
+    }
+  }
+  double* o = part + ((long)y * OCT_SLICES + blockIdx.x) * 15;
+  for (int a = 0; a < 6; ++a) {
+    const double t = block_sum256(s[a], lds);
+    if (threadIdx.x == 0) o[a] = t;
+  }
+}
+
+// Big cells, pass 2: per slice the moment about the cell's centre of mass,
+// which every block derives from pass 1's slices summed in slice order.
+__global__ void __launch_bounds__(256) k_oct_big_sum2(const OctCell* __restrict__ cells,
+                                                      const int* __restrict__ big,
+                                                      const int* __restrict__ n_big,
+                                                      const LeafSeg* __restrict__ segs,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[4];
+  const int y = blockIdx.y;
+  if (y >= *n_big) return;
+  const OctCell cell = cells[big[y]];
+  const double* p = part + (long)y * OCT_SLICES * 15;
+  double com[3];
+  for (int a = 0; a < 3; ++a) {
+    double t = 0;
+    for (int x = 0; x < OCT_SLICES; ++x) t += p[15 * x + a];
+    com[a] = t * (1.0 / cell.count);
+  }
+  const long b = cell.start + (long)cell.count * blockIdx.x / OCT_SLICES;
+  const long e = cell.start + (long)cell.count * (blockIdx.x + 1) / OCT_SLICES;
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long j = b + threadIdx.x; j < e; j += 256) {
+    const LeafSeg q = segs[j];
+    double ds[3], sl[3];
+    for (int a = 0; a < 3; ++a) {
+      ds[a] = q.b[a] - q.a[a];
+      sl[a] = q.a[a] - com[a];
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) M[3 * a + c] += ds[a] * sl[c];
+  }
+  // pass 1 wrote fields 0..5 of every slice, this pass writes 6..14 of its own
+  double* o = part + ((long)y * OCT_SLICES + blockIdx.x) * 15;
+  for (int a = 0; a < 9; ++a) {
+    const double t = block_sum256(M[a], lds);
+    if (threadIdx.x == 0) o[6 + a] = t;
+  }
+}
+
+// Big cells: slices summed in order into the cell's moments (one thread each).
+__global__ void __launch_bounds__(64) k_oct_big_fin(const OctCell* __restrict__ cells,
+                                                    const int* __restrict__ big,
+                                                    const int* __restrict__ n_big,
+                                                    const double* __restrict__ part,
+                                                    double* __restrict__ mom) {
+  const int y = blockIdx.x * 64 + threadIdx.x;
+  if (y >= *n_big) return;
+  const int c = big[y];
+  const double* p = part + (long)y * OCT_SLICES * 15;
+  double v[15];
+  for (int a = 0; a < 15; ++a) {
+    double t = 0;
+    for (int x = 0; x < OCT_SLICES; ++x) t += p[15 * x + a];
+    v[a] = t;
+  }
+  double* o = mom + 15L * c;
+  for (int a = 0; a < 3; ++a) {
+    o[a] = v[a] * (1.0 / cells[c].count);
+    o[3 + a] = v[3 + a];
+  }
+  for (int a = 0; a < 9; ++a) o[6 + a] = v[6 + a];
 }
 
 // 5a. Subtree sizes, one level at a time from the deepest.
