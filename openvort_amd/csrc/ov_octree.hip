@@ -585,6 +585,20 @@ __device__ inline void oct_multipole32(const OctNode& nd, const double r[3], boo
   acc[2] += (double)v[2];
 }
 
+// Opening test of octree.c:287-295: accept when d > 0 and Lmax / d < theta
+// (d = minimum-image distance to the centre of mass).  Decided on squares
+// (Lmax^2 against theta^2 d^2) outside a 1e-12 relative band, where the
+// rounding of either form cannot change the answer; inside it, the
+// reference's square root and division decide.
+__device__ __forceinline__ bool oct_accept(const double dv[3], double lmax, double theta) {
+  const double d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+  const double t2 = theta * theta * d2, l2 = lmax * lmax;
+  if (l2 < t2 * (1.0 - 1e-12)) return true;   // t2 > 0 here, so d > 0
+  if (l2 > t2 * (1.0 + 1e-12)) return false;
+  const double d = sqrt(d2);
+  return d > 0 && lmax / d < theta;
+}
+
 // Grid: blockIdx.x = group of 4 target tiles, blockIdx.y = image.
 // MIXED: multipoles in FP32 (leaf pairs and all decisions in FP64).
 template <bool MIXED>
@@ -626,8 +640,7 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
         double cw[3];
         pwrap(rs, nd.com, A.op.box, cw);
         const double dv[3] = {rs[0] - cw[0], rs[1] - cw[1], rs[2] - cw[2]};
-        const double d = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-        if (d > 0 && nd.lmax / d < A.op.theta) {
+        if (oct_accept(dv, nd.lmax, A.op.theta)) {
           if (MIXED)
             oct_multipole32(nd, rs, A.op.dipole != 0, acc);
           else
