@@ -59,7 +59,8 @@ def err_stats(x, y):
 
 # octree build passes timed per evaluation (the radix sort runs inside hipcub)
 OCT_BUILD = ("k_oct_keys", "k_gather_u64", "k_oct_leafsegs", "k_oct_count", "k_oct_emit",
-             "k_oct_moments", "k_oct_size", "k_oct_pre", "k_oct_nodes")
+             "k_oct_moments", "k_oct_big_sum1", "k_oct_big_sum2", "k_oct_big_fin", "k_oct_size",
+             "k_oct_pre", "k_oct_nodes")
 
 
 def tree_hbm_bytes():
@@ -295,6 +296,7 @@ def main():
             f"err {out['err_vels_mixed_vs_fp64']}")
 
     # ---- Barnes-Hut octree vs the FP64 direct sum (seeded 4096-target sample)
+    tree_samples = {}   # theta -> (octree entry, GPU FP64 reference-multipole vels on the sample)
     if args.tree > 0:
         rng_t = np.random.default_rng(2002)
         samp = np.sort(rng_t.choice(tg.active, size=min(4096, n), replace=False))
@@ -320,6 +322,8 @@ def main():
                              "build_kernels_ms": {kn: round(ctx.kernel_stats(kn)[1], 3) for kn in OCT_BUILD}})
                 if args.fp64 > 0:
                     tree[-1]["err_vels_vs_fp64_direct_sample"] = err_stats(x.vels[samp], b.vels[samp])
+                if mp == T.MP_REFERENCE and prec == T.PREC_FP64:
+                    tree_samples[th] = (len(tree) - 1, x.vels[samp].copy())
                 log(f"tree {pname} {tree[-1]['multipole']} theta {th}: {ms:.1f} ms, "
                     f"err {tree[-1].get('err_vels_vs_fp64_direct_sample')}")
         hbm = tree_hbm_bytes()
@@ -413,6 +417,18 @@ def main():
         if args.fp64 > 0:
             out["err_vels_fp64_vs_cpu_sample"] = err_stats(b.vels[idx], vel_c)
             out["err_vels_mixed_vs_cpu_sample"] = err_stats(a.vels[idx], vel_c)
+        # the reference's own octree on the octree section's 4096-target sample
+        # (build + evaluation per theta), against the GPU tree's velocities
+        if tree_samples and kind == "reference":
+            for th, (entry, vel_g) in tree_samples.items():
+                pr = T.default_params(precision=T.PREC_FP64, use_BH=1, BH_resolution=th, **common)
+                c0 = time.time()
+                _, vel_r = eng.velocities_sample(c, 0.0, pr, samp.astype(np.int32))
+                out["octree"][entry]["cpu_reference"] = {
+                    "threads": threads, "targets": int(len(samp)),
+                    "build_plus_eval_s": round(time.time() - c0, 2),
+                    "err_vels_gpu_vs_reference_tree": err_stats(vel_g, vel_r)}
+                log(f"cpu reference octree theta {th}: {out['octree'][entry]['cpu_reference']}")
 
     ctx.close()
     res = {

@@ -280,7 +280,7 @@ __device__ inline double wave_sum(double v) {
 // rounding only).
 // Cells of more than OCT_BIG points are only listed here (big, n_big) and
 // summed by the k_oct_big_* kernels, OCT_SLICES blocks per cell.
-constexpr int OCT_BIG = 8192;
+constexpr int OCT_BIG = 2048;
 constexpr int OCT_SLICES = 64;
 
 __global__ void __launch_bounds__(256) k_oct_moments(const OctCell* __restrict__ cells, int ncell,
