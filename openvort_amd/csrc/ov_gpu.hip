@@ -99,7 +99,8 @@ struct ov_ctx {
   OctCell* t_cells = nullptr;
   int *t_nchild = nullptr, *t_offs = nullptr, *t_size = nullptr, *t_pre = nullptr;
   double* t_mom = nullptr;
-  OctNode* t_nodes = nullptr;
+  OctNode* t_nodes = nullptr;   // tree_cells + 1
+  OctMom* t_nmom = nullptr;
   long t_ccap = 0;
   void* t_cub = nullptr;
   size_t t_cub_bytes = 0;
@@ -503,7 +504,8 @@ int tree_point_buffers(ov_ctx* ctx, long need) {
   if ((rc = dalloc(ctx, (void**)&ctx->t_vals, sizeof(int) * cap))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->t_vals2, sizeof(int) * cap))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->t_sorted, sizeof(int) * cap))) return rc;
-  if ((rc = dalloc(ctx, (void**)&ctx->t_segs, sizeof(LeafSeg) * cap))) return rc;
+  // one entry past the points for the traversal's look-ahead load
+  if ((rc = dalloc(ctx, (void**)&ctx->t_segs, sizeof(LeafSeg) * (cap + 1)))) return rc;
   if (!ctx->t_maxd && (rc = dalloc(ctx, (void**)&ctx->t_maxd, 256))) return rc;
   if (!ctx->t_stats && (rc = dalloc(ctx, (void**)&ctx->t_stats, 256))) return rc;
   ctx->t_pcap = cap;
@@ -528,7 +530,8 @@ int tree_cell_buffers(ov_ctx* ctx, long need, long keep) {
   if ((rc = dalloc(ctx, (void**)&ctx->t_size, sizeof(int) * cap))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->t_pre, sizeof(int) * cap))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->t_mom, sizeof(double) * 15 * cap))) return rc;
-  if ((rc = dalloc(ctx, (void**)&ctx->t_nodes, sizeof(OctNode) * cap))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_nodes, sizeof(OctNode) * (cap + 1)))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->t_nmom, sizeof(OctMom) * cap))) return rc;
   ctx->t_ccap = cap;
   return OV_OK;
 }
@@ -683,9 +686,9 @@ int build_tree(ov_ctx* ctx, const double* X) {
                      (const OctCell*)ctx->t_cells, a, b, (const int*)ctx->t_size, ctx->t_pre)))
       return rc;
   }
-  if ((rc = launch(ctx, "k_oct_nodes", k_oct_nodes, dim3((unsigned)((ncell + 255) / 256)), dim3(256),
+  if ((rc = launch(ctx, "k_oct_nodes", k_oct_nodes, dim3((unsigned)((ncell + 256) / 256)), dim3(256),
                    (const OctCell*)ctx->t_cells, ncell, (const double*)ctx->t_mom,
-                   (const int*)ctx->t_size, (const int*)ctx->t_pre, ctx->t_nodes)))
+                   (const int*)ctx->t_size, (const int*)ctx->t_pre, ctx->t_nodes, ctx->t_nmom)))
     return rc;
   return OV_OK;
 }
@@ -712,6 +715,7 @@ int tree_sum(ov_ctx* ctx, const double* X) {
   A.n = n;
   A.n_tgt_tiles = n_tgt;
   A.nodes = ctx->t_nodes;
+  A.nmom = ctx->t_nmom;
   A.n_nodes = (int)ctx->tree_cells;
   A.segs = ctx->t_segs;
   A.img = ctx->img;
@@ -1276,7 +1280,7 @@ void ov_ctx_destroy(ov_ctx* ctx) {
   void* tb[] = {ctx->t_order, ctx->t_k[0], ctx->t_k[1], ctx->t_k[2], ctx->t_ks[0], ctx->t_ks[1],
                 ctx->t_ks[2], ctx->t_ktmp, ctx->t_kout, ctx->t_vals, ctx->t_vals2, ctx->t_sorted,
                 ctx->t_maxd, ctx->t_segs, ctx->t_cells, ctx->t_nchild, ctx->t_offs, ctx->t_size,
-                ctx->t_pre, ctx->t_mom, ctx->t_nodes, ctx->t_cub, ctx->t_stats, ctx->t_big,
+                ctx->t_pre, ctx->t_mom, ctx->t_nodes, ctx->t_nmom, ctx->t_cub, ctx->t_stats, ctx->t_big,
                 ctx->t_big_part};
   for (void* b : tb)
     if (b) hipFree(b);

@@ -35,12 +35,17 @@ struct OctCell {                   // build-time cell, breadth-first order
   int child_first, n_child;
 };
 
-struct OctNode {                   // traversal node, preorder
+// Traversal node in preorder, split in two: what the walk reads at every
+// visit (one 64-byte line, so one scalar load) and the moments, read only
+// where a node is taken as a multipole.
+struct alignas(64) OctNode {
   double com[3];
-  double gam[3];
-  double M[9];
   double lmax, lmin;
   int n, first, skip, pad;
+};
+struct OctMom {
+  double gam[3];
+  double M[9];
 };
 
 struct LeafSeg {                   // segment of a sorted point
@@ -457,17 +462,27 @@ __global__ void __launch_bounds__(256) k_oct_nodes(const OctCell* __restrict__ c
                                                    const double* __restrict__ mom,
                                                    const int* __restrict__ size,
                                                    const int* __restrict__ pre,
-                                                   OctNode* __restrict__ nodes) {
+                                                   OctNode* __restrict__ nodes,
+                                                   OctMom* __restrict__ nmom) {
   const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncell) return;
-  const OctCell cell = cells[c];
+  if (c > ncell) return;
   OctNode nd;
+  if (c == ncell) {   // past the last node: read by the walk's look-ahead only
+    nd.com[0] = nd.com[1] = nd.com[2] = 0;
+    nd.lmax = nd.lmin = 0;
+    nd.n = nd.first = nd.pad = 0;
+    nd.skip = 1;
+    nodes[ncell] = nd;
+    return;
+  }
+  const OctCell cell = cells[c];
+  OctMom mm;
   const double* m = mom + 15L * c;
   for (int a = 0; a < 3; ++a) {
     nd.com[a] = m[a];
-    nd.gam[a] = m[3 + a];
+    mm.gam[a] = m[3 + a];
   }
-  for (int a = 0; a < 9; ++a) nd.M[a] = m[6 + a];
+  for (int a = 0; a < 9; ++a) mm.M[a] = m[6 + a];
   nd.lmax = oct_max_len(cell.lo, cell.hi);
   nd.lmin = oct_min_len(cell.lo, cell.hi);
   nd.n = cell.count;
@@ -475,6 +490,7 @@ __global__ void __launch_bounds__(256) k_oct_nodes(const OctCell* __restrict__ c
   nd.skip = size[c];
   nd.pad = 0;
   nodes[pre[c]] = nd;
+  nmom[pre[c]] = mm;
 }
 
 // ---------------------------------------------------------------------------
@@ -510,7 +526,8 @@ struct OctArgs {
   long cap;
   int n;
   int n_tgt_tiles;
-  const OctNode* nodes;
+  const OctNode* nodes;   // n_nodes + 1 (the last one past the end)
+  const OctMom* nmom;
   int n_nodes;
   const LeafSeg* segs;
   const ImgDev* img;
@@ -525,9 +542,9 @@ struct OctArgs {
 // caller.  Reference form: Gamma x R/|R|^3 + (M_21, 0, 0)/|R|^3 + (u x R)/|R|^5,
 // u_i = M_i2 R_2; dipole form: Gamma x R/|R|^3 - eps:M/|R|^3 + 3 (M R) x R/|R|^5.
 // R = r - centre, not minimum-imaged.
-__device__ inline void oct_multipole(const OctNode& nd, const double r[3], bool dipole,
-                                     double acc[3]) {
-  const double R[3] = {r[0] - nd.com[0], r[1] - nd.com[1], r[2] - nd.com[2]};
+__device__ inline void oct_multipole(const double com[3], const OctMom& nd, const double r[3],
+                                     bool dipole, double acc[3]) {
+  const double R[3] = {r[0] - com[0], r[1] - com[1], r[2] - com[2]};
   const double R2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
   const double ir = rsq64(R2);
   const double i3 = ir * ir * ir;
@@ -555,9 +572,9 @@ __device__ inline void oct_multipole(const OctNode& nd, const double r[3], bool 
 
 // FP32 form of oct_multipole: R = r - centre in FP64, everything after in
 // FP32 (the node's moments rounded once).
-__device__ inline void oct_multipole32(const OctNode& nd, const double r[3], bool dipole,
-                                       double acc[3]) {
-  const float R[3] = {(float)(r[0] - nd.com[0]), (float)(r[1] - nd.com[1]), (float)(r[2] - nd.com[2])};
+__device__ inline void oct_multipole32(const double com[3], const OctMom& nd, const double r[3],
+                                       bool dipole, double acc[3]) {
+  const float R[3] = {(float)(r[0] - com[0]), (float)(r[1] - com[1]), (float)(r[2] - com[2])};
   const float R2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
   const float ir = __builtin_amdgcn_rsqf(R2);
   const float i3 = ir * ir * ir;
@@ -621,19 +638,29 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
   long leaf_pairs = 0, mps = 0, visits = 0;
   int next = 0;   // this lane resumes at node >= next
   int node = 0;
+  // The walk's next node is node + 1 (some lane opens this one) or
+  // node + skip: both are loaded before this node is decided, so their
+  // latency overlaps the decision.
+  OctNode nd = A.nodes[0];
   while (node < A.n_nodes) {
-    const OctNode nd = A.nodes[node];
+    const OctNode n_in = A.nodes[node + 1];
+    const OctNode n_over = A.nodes[node + nd.skip];
     ++visits;
     bool open = false;
     if (node >= next) {
       if (nd.n == 0) {
         next = node + nd.skip;
       } else if (nd.n == 1 || nd.lmin < A.op.grain) {
+        // segs holds one entry past the last sorted point, so the look-ahead
+        // load of segment j + 1 stays in bounds
+        LeafSeg q = A.segs[nd.first];
         for (int j = nd.first; j < nd.first + nd.n; ++j) {
-          const LeafSeg q = A.segs[j];
-          if (q.m == skip_id || q.fwd == skip_id) continue;
-          oct_segment(q.a, q.b, rs, acc);
-          ++leaf_pairs;
+          const LeafSeg qn = A.segs[j + 1];
+          if (q.m != skip_id && q.fwd != skip_id) {
+            oct_segment(q.a, q.b, rs, acc);
+            ++leaf_pairs;
+          }
+          q = qn;
         }
         next = node + nd.skip;
       } else {
@@ -641,10 +668,11 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
         pwrap(rs, nd.com, A.op.box, cw);
         const double dv[3] = {rs[0] - cw[0], rs[1] - cw[1], rs[2] - cw[2]};
         if (oct_accept(dv, nd.lmax, A.op.theta)) {
+          const OctMom mm = A.nmom[node];
           if (MIXED)
-            oct_multipole32(nd, rs, A.op.dipole != 0, acc);
+            oct_multipole32(nd.com, mm, rs, A.op.dipole != 0, acc);
           else
-            oct_multipole(nd, rs, A.op.dipole != 0, acc);
+            oct_multipole(nd.com, mm, rs, A.op.dipole != 0, acc);
           ++mps;
           next = node + nd.skip;
         } else {
@@ -653,7 +681,13 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
         }
       }
     }
-    node = __builtin_amdgcn_ballot_w64(open) != 0 ? node + 1 : node + nd.skip;
+    if (__builtin_amdgcn_ballot_w64(open) != 0) {
+      node = node + 1;
+      nd = n_in;
+    } else {
+      node = node + nd.skip;
+      nd = n_over;
+    }
   }
   if (im.axis >= 0) acc[im.axis] = -acc[im.axis];
   if (valid) {
