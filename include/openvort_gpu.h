@@ -72,11 +72,53 @@ typedef struct {                                        /* tangle.h:82-106 */
 
 /* ---- parameters: the reference globals + flow descriptors --------------- */
 
-/* Imposed flows (external_velocity.c:37-50, 176-188).  Only the flows the
- * workloads use run on the device; any other type is OV_E_UNSUPPORTED. */
-enum { OV_FLOW_NONE = 0,     /* "no flow"  external_velocity.c:176-182 */
-       OV_FLOW_SIMPLE = 1 }; /* "simple"   external_velocity.c:184-188 */
-typedef struct { int type; double v[3]; } ov_flow;
+/* Imposed flows: every entry of the reference's flow table v_confs[]
+ * (external_velocity.c:85-174; "no flow" and "simple" first, the others in
+ * table order), evaluated per point at the
+ * point's position and the evaluation's time (tangle.c:601, 662).
+ * Parameters, by the names of the table's v_params:
+ *   v     "external_v"                                     SIMPLE, SPHERICAL_AND_SIMPLE
+ *   a[0..3] the scalar parameters in table order:
+ *     SPHERICAL, SPHERICAL_AND_SIMPLE, CYLINDRICAL: strength, cutoff
+ *     OSCILLATING:        strength, frequency      (w = "wave", q = "polarization")
+ *     OSCILLATING_SLIT:   strength, frequency, cutoff
+ *     COSCOS, COSCOS_DIVFREE, COS_DIVFREE:          k, v0, l
+ *     SIMPLE_SHEAR:       rate, v0
+ *     ROTOSC:             freq, amp
+ *     DG:                 rot, vz
+ *     ROTOSC_BOUNDARY:    freq, amp, delta
+ *     CONSTANT_ACCELERATION: accel
+ *     PARABOLIC:          H, amp
+ * Each flow computes what the reference's function computes, quirks included:
+ * "oscillating slit" ignores strength (external_velocity.c:296-322),
+ * "rotosc-boundary" runs get_v_rotosc (the table entry, :156-157), so delta is
+ * unused.  "spherical and simple" returns in the reference before writing its
+ * result (get_v_simple returns 1, :225-241), leaving the caller's vector
+ * uninitialised; here it is the sum of the two flows, what the function
+ * computes before that return. */
+enum { OV_FLOW_NONE = 0,                  /* "no flow" external_velocity.c:176-182 */
+       OV_FLOW_SIMPLE = 1,                /* "simple"               :184-188 */
+       OV_FLOW_SPHERICAL = 2,             /* "spherical"            :197-223 */
+       OV_FLOW_SPHERICAL_AND_SIMPLE = 3,  /* "spherical and simple" :225-241 */
+       OV_FLOW_CYLINDRICAL = 4,           /* "cylindrical"          :243-270 */
+       OV_FLOW_OSCILLATING = 5,           /* "oscillating"          :272-294 */
+       OV_FLOW_OSCILLATING_SLIT = 6,      /* "oscillating slit"     :296-322 */
+       OV_FLOW_COSCOS = 7,                /* "coscos"               :324-344 */
+       OV_FLOW_COSCOS_DIVFREE = 8,        /* "coscos-divfree"       :347-370 */
+       OV_FLOW_COS_DIVFREE = 9,           /* "cos-divfree"          :372-394 */
+       OV_FLOW_SIMPLE_SHEAR = 10,         /* "simple-shear"         :396-413 */
+       OV_FLOW_ROTOSC = 11,               /* "rotosc"               :415-433 */
+       OV_FLOW_DG = 12,                   /* "DG"                   :438-457 */
+       OV_FLOW_ROTOSC_BOUNDARY = 13,      /* "rotosc-boundary" (runs get_v_rotosc) */
+       OV_FLOW_CONSTANT_ACCELERATION = 14,/* "constant acceleration" :483-495 */
+       OV_FLOW_PARABOLIC = 15,            /* "parabolic"            :497-512 */
+       OV_FLOW_COUNT = 16 };
+typedef struct {
+  int type;          /* OV_FLOW_* */
+  double v[3];       /* "external_v" */
+  double a[4];       /* scalar parameters, table order */
+  double w[3], q[3]; /* "wave", "polarization" */
+} ov_flow;
 
 enum { OV_PREC_FP64 = 0,     /* every pair term in FP64 */
        OV_PREC_MIXED = 1 };  /* FP32 far field, FP64 near field, LIA, RK */

@@ -119,7 +119,9 @@ __device__ void local_velocity(const D3 p[5][3], const D3 u[3], const BoxDev& bo
   dcross(t, nm, lia);
   for (int i = 0; i < 3; ++i) lia[i] = lia[i] * beta;
   D3 vs[3];
-  for (int i = 0; i < 3; ++i) vs[i] = lia[i] + dc(ph.vs_ext[i]) + u[i];
+  // imposed flows: constant ("no flow", "simple") only; the entry points
+  // reject the others
+  for (int i = 0; i < 3; ++i) vs[i] = lia[i] + dc(ph.vs.type == OV_FLOW_SIMPLE ? ph.vs.v[i] : 0.0) + u[i];
   for (int i = 0; i < 3; ++i) vel[i] = vs[i];
   if (ph.hyperfriction) {
     const double spp = sqrt(nm[0].v * nm[0].v + nm[1].v * nm[1].v + nm[2].v * nm[2].v);
@@ -131,7 +133,7 @@ __device__ void local_velocity(const D3 p[5][3], const D3 u[3], const BoxDev& bo
   }
   if (ph.use_mf) {
     D3 dv[3], a1[3], a2[3];
-    for (int i = 0; i < 3; ++i) dv[i] = dc(ph.vn_ext[i]) - vs[i];
+    for (int i = 0; i < 3; ++i) dv[i] = dc(ph.vn.type == OV_FLOW_SIMPLE ? ph.vn.v[i] : 0.0) - vs[i];
     dcross(t, dv, a1);
     dcross(t, a1, a2);
     for (int i = 0; i < 3; ++i) vel[i] = vel[i] + a1[i] * ph.alpha - a2[i] * ph.alpha_p;

@@ -367,11 +367,12 @@ __global__ void __launch_bounds__(256) kb_reduce(const double* __restrict__ upar
   store3(bs, cap, s, a);
 }
 
-__global__ void __launch_bounds__(256) kb_epilogue(const double* __restrict__ T,
+__global__ void __launch_bounds__(256) kb_epilogue(const double* __restrict__ P,
+                                                   const double* __restrict__ T,
                                                    const double* __restrict__ Nn,
                                                    const double* __restrict__ lia,
                                                    const double* __restrict__ bs, long cap,
-                                                   const int* __restrict__ tid,
+                                                   double time, const int* __restrict__ tid,
                                                    const TangleDev* __restrict__ tg,
                                                    double* __restrict__ vs_out,
                                                    double* __restrict__ vel_out,
@@ -380,7 +381,7 @@ __global__ void __launch_bounds__(256) kb_epilogue(const double* __restrict__ T,
   if (s >= ntot) return;
   const int k = tid[s];
   if (k < 0) return;
-  epilogue_point(s, T, Nn, lia, bs, !tg[k].lia_only, cap, tg[k].ph, vs_out, vel_out, flag);
+  epilogue_point(s, P, T, Nn, lia, bs, !tg[k].lia_only, cap, time, tg[k].ph, vs_out, vel_out, flag);
 }
 
 __global__ void __launch_bounds__(256) kb_wrap(double* __restrict__ X0, long cap,

@@ -16,6 +16,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "../../include/openvort_gpu.h"
+
 namespace ov {
 
 constexpr int TILE = 64;          // entries per tile = wavefront width
@@ -79,11 +81,12 @@ struct NearRec {
 };
 
 // Physics block of the per-point epilogue (reference globals,
-// vortex_constants.c:24-95; flows external_velocity.c:176-188).
+// vortex_constants.c:24-95; imposed flows vs_conf, vn_conf,
+// external_velocity.c:76-78).
 struct PhysDev {
   double kappa, a0, alpha, alpha_p, dl_max;
   double max_curv, hyperalpha;
-  double vs_ext[3], vn_ext[3];
+  ov_flow vs, vn;
   int lia_only, use_mf, hyperfriction, pad;
 };
 

@@ -59,6 +59,7 @@ struct ov_ctx {
   double rnear = 0.02;
   int wall_periodic_mask = 0;
   bool lia_only = false;
+  bool flow_time_dep = false;   // an imposed flow depends on t (no graph replay)
 
   // device buffers (planes of `cap` doubles)
   double *X0 = nullptr, *Xs = nullptr, *K[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -273,9 +274,9 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   if (t->N < 0) return fail(ctx, OV_E_INVALID, "negative N");
   if (t->N > 0 && (!t->vnodes || !t->connections || !t->status))
     return fail(ctx, OV_E_INVALID, "null tangle arrays");
-  if (p->vs.type < OV_FLOW_NONE || p->vs.type > OV_FLOW_SIMPLE || p->vn.type < OV_FLOW_NONE ||
-      p->vn.type > OV_FLOW_SIMPLE)
-    return fail(ctx, OV_E_UNSUPPORTED, "only the 'no flow' and 'simple' flows run on the device");
+  for (const ov_flow* f : {&p->vs, &p->vn, &p->vb})
+    if (f->type < OV_FLOW_NONE || f->type >= OV_FLOW_COUNT)
+      return fail(ctx, OV_E_INVALID, "unknown flow type %d", f->type);
   if (t->bimg.n < 0 || t->bimg.n > MAX_IMAGES - 1 || (t->bimg.n > 0 && !t->bimg.images))
     return fail(ctx, OV_E_INVALID, "bad image list (n=%d)", t->bimg.n);
   const int N = t->N;
@@ -374,10 +375,10 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   ph.dl_max = p->dl_max;
   ph.max_curv = p->max_curvature_scale;
   ph.hyperalpha = p->hyperalpha;
-  for (int i = 0; i < 3; ++i) {
-    ph.vs_ext[i] = p->vs.type == OV_FLOW_SIMPLE ? p->vs.v[i] : 0.0;
-    ph.vn_ext[i] = p->vn.type == OV_FLOW_SIMPLE ? p->vn.v[i] : 0.0;
-  }
+  ph.vs = p->vs;
+  ph.vn = p->vn;
+  ctx->flow_time_dep = flow_time_dependent(p->vs.type) ||
+                       (p->use_mutual_friction && flow_time_dependent(p->vn.type));
   ph.lia_only = p->LIA_only;
   ph.use_mf = p->use_mutual_friction;
   ph.hyperfriction = p->hyperfriction;
@@ -746,7 +747,7 @@ int tree_sum(ov_ctx* ctx, const double* X) {
 // One velocity evaluation at positions X: geometry (or the caller's s', s''
 // already in Tg/Ng), the direct sum, the epilogue.
 int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, double* dxi,
-             double* vs, double* vel) {
+             double* vs, double* vel, double time) {
   const int n = ctx->n;
   const long cap = ctx->cap;
   int rc;
@@ -825,9 +826,9 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
   } else {
     ctx->last_interactions = 0;
   }
-  if ((rc = launch(ctx, "k_epilogue", k_epilogue, grid1(n), dim3(256), (const double*)Tg,
+  if ((rc = launch(ctx, "k_epilogue", k_epilogue, grid1(n), dim3(256), X, (const double*)Tg,
                    (const double*)Ng, (const double*)ctx->lia, (const double*)ctx->bs, have_bs, n,
-                   cap, ctx->ph, vs, vel, ctx->flag)))
+                   cap, time, ctx->ph, vs, vel, ctx->flag)))
     return rc;
   return OV_OK;
 }
@@ -839,7 +840,8 @@ int copy_planes(ov_ctx* ctx, double* dst, const double* src) {
   return OV_OK;
 }
 
-// update_tangle(t) + rk4_step(t, dt) on the device-resident state in X0.
+// update_tangle(t) + rk4_step(t, dt) on the device-resident state in X0;
+// the evaluations run at t, t + dt/2, t + dt/2, t + dt (vortex_dynamics.c:79-92).
 // With caller_k1, K[0] already holds k1 and the t-geometry is not produced.
 // With `store` (8 plane sets), the step's start positions, k1..k3 and the
 // Biot-Savart sums of the four evaluations are kept there, in the order
@@ -852,17 +854,17 @@ int rk4_device(ov_ctx* ctx, double t, double dt, int rk_variant, bool caller_k1,
   int rc;
   if (store && (rc = copy_planes(ctx, store, ctx->X0))) return rc;
   if (!caller_k1) {
-    if ((rc = evaluate(ctx, ctx->X0, true, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0])))
+    if ((rc = evaluate(ctx, ctx->X0, true, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0], t)))
       return rc;
     if (store && (rc = copy_planes(ctx, store + 4 * ps, ctx->bs))) return rc;
   }
   const double h[3] = {dt / 2, dt / 2, rk_variant == OV_RK_CLASSIC ? dt : dt / 2};
-  (void)t;
+  const double ts[3] = {t + dt / 2, t + dt / 2, t + dt};
   for (int st = 0; st < 3; ++st) {
     if ((rc = launch(ctx, "k_stage", k_stage, grid1(n), dim3(256), (const double*)ctx->X0,
                      (const double*)ctx->K[st], h[st], n, cap, ctx->Xs)))
       return rc;
-    if ((rc = evaluate(ctx, ctx->Xs, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[st + 1])))
+    if ((rc = evaluate(ctx, ctx->Xs, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[st + 1], ts[st])))
       return rc;
     if (store && (rc = copy_planes(ctx, store + (5 + st) * ps, ctx->bs))) return rc;
   }
@@ -943,7 +945,7 @@ long batch_unit_bound(const ov_ctx* ctx, int U) {
 
 // One velocity evaluation of every tangle of the batch at positions X.
 int evaluate_batch(ov_ctx* ctx, const double* X, double* Tg, double* Ng, double* dxi, double* vs,
-                   double* vel, float* direct_ms) {
+                   double* vel, float* direct_ms, double time) {
   const long cap = ctx->cap;
   const int ntot = ctx->b_ntot, rec_tot = ctx->b_rec_tot, n_tgt = ctx->b_n_tgt;
   const int nb = (ntot + 255) / 256, nbu = (n_tgt + 255) / 256;
@@ -1031,24 +1033,26 @@ int evaluate_batch(ov_ctx* ctx, const double* X, double* Tg, double* Ng, double*
                    (const int*)ctx->b_ucnt, (const int*)ctx->b_urank, (const int*)ctx->b_ubsum,
                    (const int*)ctx->b_tid, cap, ctx->bs, ntot)))
     return rc;
-  return launch(ctx, "kb_epilogue", kb_epilogue, dim3(nb), dim3(256), (const double*)Tg, (const double*)Ng,
-                (const double*)ctx->lia, (const double*)ctx->bs, cap, (const int*)ctx->b_tid,
-                (const TangleDev*)ctx->b_dtg, vs, vel, ctx->flag, ntot);
+  return launch(ctx, "kb_epilogue", kb_epilogue, dim3(nb), dim3(256), X, (const double*)Tg,
+                (const double*)Ng, (const double*)ctx->lia, (const double*)ctx->bs, cap, time,
+                (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, vs, vel, ctx->flag, ntot);
 }
 
 // update_tangle + rk4_step of every tangle of the batch (X0 device-resident).
-int rk4_batch(ov_ctx* ctx, double dt, int rk_variant, bool enforce) {
+int rk4_batch(ov_ctx* ctx, double t, double dt, int rk_variant, bool enforce) {
   const long cap = ctx->cap;
   const int ntot = ctx->b_ntot;
   int rc;
-  if ((rc = evaluate_batch(ctx, ctx->X0, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0], nullptr)))
+  if ((rc = evaluate_batch(ctx, ctx->X0, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0], nullptr, t)))
     return rc;
   const double h[3] = {dt / 2, dt / 2, rk_variant == OV_RK_CLASSIC ? dt : dt / 2};
+  const double ts[3] = {t + dt / 2, t + dt / 2, t + dt};
   for (int st = 0; st < 3; ++st) {
     if ((rc = launch(ctx, "k_stage", k_stage, grid1(ntot), dim3(256), (const double*)ctx->X0,
                      (const double*)ctx->K[st], h[st], ntot, cap, ctx->Xs)))
       return rc;
-    if ((rc = evaluate_batch(ctx, ctx->Xs, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[st + 1], nullptr)))
+    if ((rc = evaluate_batch(ctx, ctx->Xs, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[st + 1], nullptr,
+                             ts[st])))
       return rc;
   }
   if ((rc = launch(ctx, "k_combine", k_combine, grid1(ntot), dim3(256), ctx->X0, (const double*)ctx->K[0],
@@ -1176,6 +1180,9 @@ int adj_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   if (rc) return rc;
   if (p->use_BH)
     return fail(ctx, OV_E_UNSUPPORTED, "the adjoint differentiates the direct sum only (use_BH=0)");
+  for (const ov_flow* f : {&p->vs, &p->vn})
+    if (f->type != OV_FLOW_NONE && f->type != OV_FLOW_SIMPLE)
+      return fail(ctx, OV_E_UNSUPPORTED, "the adjoint takes the 'no flow' and 'simple' flows only");
   if (ctx->n == 0) return OV_OK;
   if ((rc = adj_buffers(ctx))) return rc;
   return upload_topology(ctx);
@@ -1379,9 +1386,8 @@ int ov_gpu_velocities(ov_ctx* ctx, ov_tangle_state* t, double time, const ov_par
     if ((rc = upload_vec3(ctx, t->tangents, ctx->T)) || (rc = upload_vec3(ctx, t->normals, ctx->Nn)))
       return rc;
   }
-  (void)time;
   if ((rc = mark_begin(ctx))) return rc;
-  if ((rc = evaluate(ctx, ctx->X0, geom, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[0]))) return rc;
+  if ((rc = evaluate(ctx, ctx->X0, geom, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[0], time))) return rc;
   int flag = 0;
   if ((rc = finish(ctx, &flag))) return rc;
   if ((rc = download_vec3(ctx, ctx->vs, t->vs)) || (rc = download_vec3(ctx, ctx->K[0], t->vels)))
@@ -1411,9 +1417,17 @@ static int rk4_common(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int
     if ((rc = upload_vec3(ctx, t->vels, ctx->K[0]))) return rc;
   }
   if ((rc = mark_begin(ctx))) return rc;
+  // step i starts at t0 + dt + ... + dt, accumulated as the reference's
+  // driver does (main.c:127)
+  std::vector<double> times(nsteps);
+  double tt = t0;
+  for (int i = 0; i < nsteps; ++i) {
+    times[i] = tt;
+    tt += dt;
+  }
   auto step = [&](int i) {
     const bool use_k1 = caller_k1 && i == 0;
-    int r = rk4_device(ctx, t0 + i * dt, dt, p->rk_variant, use_k1);
+    int r = rk4_device(ctx, times[i], dt, p->rk_variant, use_k1);
     if (!r && (flags & OV_ENFORCE_BOUNDARIES))
       r = launch(ctx, "k_wrap", k_wrap, grid1(ctx->n), dim3(256), ctx->X0, ctx->n, ctx->cap, ctx->box,
                  ctx->wall_periodic_mask);
@@ -1421,9 +1435,10 @@ static int rk4_common(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int
   };
   // The first step runs eagerly (it sizes every buffer); the direct-sum steps
   // after it are one captured step replayed as a graph, which removes the
-  // per-launch overhead that dominates small tangles.  Profiling and the
-  // octree (host-read cell counts) keep eager launches.
-  const bool graphs = !ctx->profile && !ctx->use_tree;
+  // per-launch overhead that dominates small tangles.  Profiling, the octree
+  // (host-read cell counts) and flows that depend on t (a graph replays its
+  // launch arguments) keep eager launches.
+  const bool graphs = !ctx->profile && !ctx->use_tree && !ctx->flow_time_dep;
   int i = 0;
   for (; i < nsteps && (i == 0 || !graphs); ++i)
     if ((rc = step(i))) return rc;
@@ -1454,7 +1469,6 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
   if (!tangles || !params || ntangles <= 0 || nsteps < 0)
     return fail(ctx, OV_E_INVALID, "batch: bad arguments");
   HIPCHK(hipSetDevice(ctx->device));
-  (void)t0;
   const int K = ntangles;
   const int prec = params[0].precision, rkv = params[0].rk_variant;
   ctx->b_tg.assign(K, TangleDev{});
@@ -1463,6 +1477,7 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
   std::vector<ImgDev> imgs;
   std::vector<int> npts(K);
   double rnear = 0;
+  bool time_dep = false;
   for (int k = 0; k < K; ++k) {
     if (params[k].use_BH) return fail(ctx, OV_E_UNSUPPORTED, "batch: octree not supported (tangle %d)", k);
     if (params[k].precision != prec || params[k].rk_variant != rkv)
@@ -1474,6 +1489,7 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
       return fail(ctx, rc, "%s", msg);
     }
     if (ctx->n == 0) return fail(ctx, OV_E_INVALID, "batch: tangle %d is empty", k);
+    time_dep = time_dep || ctx->flow_time_dep;
     TangleDev& t = ctx->b_tg[k];
     t.box = ctx->box;
     t.ph = ctx->ph;
@@ -1612,7 +1628,8 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
         ctx->b_U = U;
         ctx->b_blocks = dev_cus * m;
         float ms = 0;
-        if ((rc = evaluate_batch(ctx, ctx->X0, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0], &ms))) return rc;
+        if ((rc = evaluate_batch(ctx, ctx->X0, ctx->T1, ctx->N1, ctx->dxi1, ctx->vs1, ctx->K[0], &ms, t0)))
+          return rc;
         tune_ms += ms;
         if (ms < best) {
           best = ms;
@@ -1627,14 +1644,21 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
   }
   ctx->b_U = it->second.first;
   ctx->b_blocks = it->second.second;
-  // one RK4 step captured as a graph, replayed nsteps times
+  // one RK4 step captured as a graph, replayed nsteps times; with flows that
+  // depend on t, eager steps at the accumulated times of main.c:127
   const bool enforce = (flags & OV_ENFORCE_BOUNDARIES) != 0;
   if ((rc = mark_begin(ctx))) return rc;
-  if (nsteps > 0) {
+  if (nsteps > 0 && time_dep) {
+    double tt = t0;
+    for (int i = 0; i < nsteps; ++i) {
+      if ((rc = rk4_batch(ctx, tt, dt, rkv, enforce))) return rc;
+      tt += dt;
+    }
+  } else if (nsteps > 0) {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    rc = rk4_batch(ctx, dt, rkv, enforce);
+    rc = rk4_batch(ctx, t0, dt, rkv, enforce);
     hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
     if (rc) {
       if (graph) hipGraphDestroy(graph);
@@ -1704,11 +1728,10 @@ int ov_gpu_velocity_vjp(ov_ctx* ctx, const ov_tangle_state* t, double time, cons
     return OV_OK;
   }
   ctx->profile = (flags & OV_PROFILE) != 0;
-  (void)time;
   if ((rc = upload_vec3(ctx, t->vnodes, ctx->X0)) || (rc = upload_vec3(ctx, cot_vels, ctx->a_W)))
     return rc;
   if ((rc = mark_begin(ctx))) return rc;
-  if ((rc = evaluate(ctx, ctx->X0, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[0])) ||
+  if ((rc = evaluate(ctx, ctx->X0, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[0], time)) ||
       (rc = velocity_vjp_dev(ctx, ctx->X0, ctx->bs, ctx->a_W, ctx->a_G)) ||
       (rc = launch(ctx, "ka_finite", ka_finite, grid1(ctx->n), dim3(256), (const double*)ctx->a_G,
                    ctx->n, ctx->cap, ctx->flag)))

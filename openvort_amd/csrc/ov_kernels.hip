@@ -6,6 +6,7 @@
 #include <math.h>
 
 #include "ov_device.h"
+#include "ov_flow.h"
 
 namespace ov {
 
@@ -944,23 +945,26 @@ __global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ parti
 // ---------------------------------------------------------------------------
 
 // Reference update_velocity tail (tangle.c:596-683):
-//   vs   = v_LIA + vs_ext + v_BS
+//   vs   = v_LIA + vs_ext(r, t) + v_BS
 //   vels = vs - [|s''| > c/dl_max] alpha_h s' x v_LIA
-//             + alpha s' x (vn - vs) - alpha' s' x (s' x (vn - vs))
-__device__ __forceinline__ void epilogue_point(int s, const double* __restrict__ T,
+//             + alpha s' x (vn(r, t) - vs) - alpha' s' x (s' x (vn(r, t) - vs))
+__device__ __forceinline__ void epilogue_point(int s, const double* __restrict__ P,
+                                               const double* __restrict__ T,
                                                const double* __restrict__ Nn,
                                                const double* __restrict__ lia,
                                                const double* __restrict__ bs, int have_bs,
-                                               long cap, const PhysDev& ph,
+                                               long cap, double time, const PhysDev& ph,
                                                double* __restrict__ vs_out,
                                                double* __restrict__ vel_out,
                                                int* __restrict__ flag) {
-  double t[3], nm[3], l[3], vs[3], v[3];
+  double r[3], t[3], nm[3], l[3], vs[3], v[3], ext[3];
+  load3(P, cap, s, r);
   load3(T, cap, s, t);
   load3(Nn, cap, s, nm);
   load3(lia, cap, s, l);
+  flow_eval(ph.vs, r, time, ext);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) vs[i] = l[i] + ph.vs_ext[i];
+  for (int i = 0; i < 3; ++i) vs[i] = l[i] + ext[i];
   if (have_bs) {
     double b[3];
     load3(bs, cap, s, b);
@@ -978,7 +982,9 @@ __device__ __forceinline__ void epilogue_point(int s, const double* __restrict__
     }
   }
   if (ph.use_mf) {
-    double dv[3] = {ph.vn_ext[0] - vs[0], ph.vn_ext[1] - vs[1], ph.vn_ext[2] - vs[2]};
+    double vn[3];
+    flow_eval(ph.vn, r, time, vn);
+    double dv[3] = {vn[0] - vs[0], vn[1] - vs[1], vn[2] - vs[2]};
     double a1[3] = {t[1] * dv[2] - t[2] * dv[1], t[2] * dv[0] - t[0] * dv[2],
                     t[0] * dv[1] - t[1] * dv[0]};
     double a2[3] = {t[1] * a1[2] - t[2] * a1[1], t[2] * a1[0] - t[0] * a1[2],
@@ -996,16 +1002,18 @@ __device__ __forceinline__ void epilogue_point(int s, const double* __restrict__
   if (bad) atomicOr(flag, 1);
 }
 
-__global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ T,
+__global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ P,
+                                                  const double* __restrict__ T,
                                                   const double* __restrict__ Nn,
                                                   const double* __restrict__ lia,
                                                   const double* __restrict__ bs, int have_bs, int n,
-                                                  long cap, PhysDev ph, double* __restrict__ vs_out,
+                                                  long cap, double time, PhysDev ph,
+                                                  double* __restrict__ vs_out,
                                                   double* __restrict__ vel_out,
                                                   int* __restrict__ flag) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  epilogue_point(s, T, Nn, lia, bs, have_bs, cap, ph, vs_out, vel_out, flag);
+  epilogue_point(s, P, T, Nn, lia, bs, have_bs, cap, time, ph, vs_out, vel_out, flag);
 }
 
 // ---------------------------------------------------------------------------

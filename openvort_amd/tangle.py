@@ -63,10 +63,66 @@ class TangleState(C.Structure):
 
 
 class Flow(C.Structure):
-    _fields_ = [("type", C.c_int), ("v", C.c_double * 3)]
+    _fields_ = [
+        ("type", C.c_int),
+        ("v", C.c_double * 3),
+        ("a", C.c_double * 4),
+        ("w", C.c_double * 3),
+        ("q", C.c_double * 3),
+    ]
 
 
+# The reference's flow table (src/external_velocity.c:85-174), in order: the
+# name of each entry, its type code and its parameters by name -> slot of
+# ov_flow ("external_v" -> v, scalars -> a[i] in table order, "wave" -> w,
+# "polarization" -> q).
+FLOWS = {
+    "no flow": (0, []),
+    "simple": (1, ["external_v"]),
+    "spherical": (2, ["strength", "cutoff"]),
+    "spherical and simple": (3, ["external_v", "strength", "cutoff"]),
+    "cylindrical": (4, ["strength", "cutoff"]),
+    "oscillating": (5, ["strength", "frequency", "wave", "polarization"]),
+    "oscillating slit": (6, ["strength", "frequency", "cutoff"]),
+    "coscos": (7, ["k", "v0", "l"]),
+    "coscos-divfree": (8, ["k", "v0", "l"]),
+    "cos-divfree": (9, ["k", "v0", "l"]),
+    "simple-shear": (10, ["rate", "v0"]),
+    "rotosc": (11, ["freq", "amp"]),
+    "DG": (12, ["rot", "vz"]),
+    "rotosc-boundary": (13, ["freq", "amp", "delta"]),
+    "constant acceleration": (14, ["accel"]),
+    "parabolic": (15, ["H", "amp"]),
+}
 FLOW_NONE, FLOW_SIMPLE = 0, 1
+FLOW_VECTORS = {"external_v": "v", "wave": "w", "polarization": "q"}
+
+
+def set_flow(f: Flow, name: str = "no flow", **params) -> Flow:
+    """Fill an ov_flow with the reference flow `name` and its parameters
+    (missing ones are 0, as the reference's table initialises them)."""
+    if name not in FLOWS:
+        raise ValueError(f"unknown flow {name!r}")
+    code, names = FLOWS[name]
+    unknown = set(params) - set(names)
+    if unknown:
+        raise ValueError(f"flow {name!r} has no parameters {sorted(unknown)}")
+    f.type = code
+    for i in range(3):
+        f.v[i] = f.w[i] = f.q[i] = 0.0
+    for i in range(4):
+        f.a[i] = 0.0
+    scalar = 0
+    for pn in names:
+        if pn in FLOW_VECTORS:
+            dst = getattr(f, FLOW_VECTORS[pn])
+            val = params.get(pn, (0.0, 0.0, 0.0))
+            for i in range(3):
+                dst[i] = float(val[i])
+        else:
+            f.a[scalar] = float(params.get(pn, 0.0))
+            scalar += 1
+    return f
 PREC_FP64, PREC_MIXED = 0, 1
 RK_REFERENCE, RK_CLASSIC = 0, 1
 MP_REFERENCE, MP_DIPOLE = 0, 1
@@ -121,13 +177,17 @@ def default_params(**kw) -> Params:
     p.near_radius = 0.0
     for name, val in kw.items():
         if name in ("vn", "vs", "vb"):
+            # None: "no flow"; a 3-vector: "simple"; (flow name, {params}) or
+            # a flow name alone: that entry of the flow table
             f = getattr(p, name)
             if val is None:
-                f.type = FLOW_NONE
+                set_flow(f)
+            elif isinstance(val, str):
+                set_flow(f, val)
+            elif isinstance(val, tuple) and len(val) == 2 and isinstance(val[0], str):
+                set_flow(f, val[0], **val[1])
             else:
-                f.type = FLOW_SIMPLE
-                for i in range(3):
-                    f.v[i] = float(val[i])
+                set_flow(f, "simple", external_v=val)
         else:
             setattr(p, name, val)
     return p

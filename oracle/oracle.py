@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(_HERE))
 
-from openvort_amd.tangle import Params, Tangle, TangleState  # noqa: E402
+from openvort_amd.tangle import Flow, Params, Tangle, TangleState  # noqa: E402
 
 ORACLE_LIB = os.path.join(_HERE, "liboracle.so")
 REF_LIB = os.path.join(_HERE, "_ref", "libovref.so")
@@ -52,6 +52,7 @@ class Oracle:
             lib.ovo_reconnection_candidates.argtypes = [P(TangleState), C.c_double, C.c_double,
                                                         P(C.c_int), C.c_long]
             lib.ovo_reconnection_candidates.restype = C.c_long
+            lib.ovo_flow.argtypes = [P(Flow), P(C.c_double), C.c_double, P(C.c_double)]
             Oracle._lib = lib
         self.lib = Oracle._lib
         self.lib.ovo_set_threads(threads)
@@ -121,6 +122,18 @@ class Oracle:
         self.lib.ovo_octree_stats(C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
 
+    def flow(self, f: Flow, r, t: float):
+        """The imposed flow f at position r, time t."""
+        return _flow_call(self.lib.ovo_flow, f, r, t)
+
+
+def _flow_call(fn, f, r, t):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(3)
+    if fn(C.byref(f), r.ctypes.data_as(P(C.c_double)), t, out.ctypes.data_as(P(C.c_double))) != 0:
+        raise ValueError(f"flow type {f.type}")
+    return out
+
 
 class Reference:
     """The reference's own C (update_tangents_normals, update_velocities,
@@ -151,6 +164,7 @@ class Reference:
             lib.ovref_reconnect.restype = C.c_int
             lib.ovref_reconnect_candidates.argtypes = [P(TangleState), C.c_double, P(C.c_int), C.c_long]
             lib.ovref_reconnect_candidates.restype = C.c_int
+            lib.ovref_flow.argtypes = [P(Flow), P(C.c_double), C.c_double, P(C.c_double)]
             Reference._lib = lib
         self.lib = Reference._lib
         self.threads = threads
@@ -210,3 +224,8 @@ class Reference:
         ts = t.cstruct()
         return self.lib.ovref_reconnect_candidates(C.byref(ts), time, pairs.ctypes.data_as(P(C.c_int)),
                                                    len(pairs))
+
+    def flow(self, f: Flow, r, t: float):
+        """The reference's own flow function (its table entry for f's type,
+        parameters set by name) at position r, time t."""
+        return _flow_call(self.lib.ovref_flow, f, r, t)

@@ -238,11 +238,117 @@ int ovo_update_tangents_normals(ov_tangle_state *t) {
 
 /* ---- external flows ------------------------------------------------------ */
 
-/* "no flow" and "simple" (external_velocity.c:176-188). */
-static int flow_eval(const ov_flow *fl, double out[3]) {
-  if (fl->type == OV_FLOW_NONE) { out[0] = out[1] = out[2] = 0; return 0; }
-  if (fl->type == OV_FLOW_SIMPLE) { for (int i = 0; i < 3; ++i) out[i] = fl->v[i]; return 0; }
-  return OV_E_UNSUPPORTED;
+/* |u| as vec3_d (vec3_maths.c:338-345): squares summed from zero. */
+static double vnorm(double x, double y, double z) {
+  double o = 0;
+  o += x * x;
+  o += y * y;
+  o += z * z;
+  return sqrt(o);
+}
+
+/* Radial source/sink field, strength/(4 pi r^2) or, about the z axis,
+ * strength/(2 pi r), damped by exp(-(cutoff/r)^2) and cut to zero below
+ * r = 1e-8 or beyond an exponent of 100 (external_velocity.c:197-223,
+ * 243-270).  cyl selects the axial form. */
+static void radial_flow(const ov_flow *fl, const double r[3], int cyl, double out[3]) {
+  const double strength = fl->a[0], cutoff = fl->a[1];
+  const double d = cyl ? sqrt(r[0] * r[0] + r[1] * r[1]) : vnorm(r[0], r[1], r[2]);
+  const double attn = (cutoff / d) * (cutoff / d);
+  if (d < 1e-8 || attn > 100) {
+    out[0] = out[1] = out[2] = 0;
+    return;
+  }
+  const double factor = exp(-attn);
+  const double u[3] = {r[0], r[1], cyl ? 0.0 : r[2]};
+  const double inv = 1 / vnorm(u[0], u[1], u[2]); /* vec3_normalize */
+  const double m = cyl ? strength / (2 * M_PI * d) * factor : strength / (4 * M_PI * d * d) * factor;
+  for (int i = 0; i < 3; ++i) out[i] = (u[i] * inv) * m;
+}
+
+/* Every flow of the reference table (external_velocity.c:85-174) at r, t;
+ * the parameter slots are those of ov_flow (include/openvort_gpu.h). */
+static int flow_eval(const ov_flow *fl, const double r[3], double t, double out[3]) {
+  const double x = r[0], y = r[1], z = r[2];
+  const double *a = fl->a;
+  switch (fl->type) {
+  case OV_FLOW_NONE: /* :176-182 */
+    out[0] = out[1] = out[2] = 0;
+    return 0;
+  case OV_FLOW_SIMPLE: /* :184-188 */
+    for (int i = 0; i < 3; ++i) out[i] = fl->v[i];
+    return 0;
+  case OV_FLOW_SPHERICAL:
+    radial_flow(fl, r, 0, out);
+    return 0;
+  case OV_FLOW_SPHERICAL_AND_SIMPLE: { /* :225-241, the sum it computes */
+    double sph[3];
+    radial_flow(fl, r, 0, sph);
+    for (int i = 0; i < 3; ++i) out[i] = sph[i] + fl->v[i];
+    return 0;
+  }
+  case OV_FLOW_CYLINDRICAL:
+    radial_flow(fl, r, 1, out);
+    return 0;
+  case OV_FLOW_OSCILLATING: { /* :272-294: pol strength cos(2 pi k.r) sin(2 pi f t) */
+    double kr = 0;
+    kr += fl->w[0] * x;
+    kr += fl->w[1] * y;
+    kr += fl->w[2] * z;
+    const double m = a[0] * cos(2 * M_PI * kr) * sin(2 * M_PI * a[1] * t);
+    for (int i = 0; i < 3; ++i) out[i] = fl->q[i] * m;
+    return 0;
+  }
+  case OV_FLOW_OSCILLATING_SLIT: { /* :296-322, strength unused */
+    const double rr = sqrt(x * x + z * z);
+    const double dir[3] = {x / rr, 0.0, z / rr};
+    const double m = cos(2 * M_PI * a[1] * t) * exp(-a[2] * a[2] / rr / rr);
+    for (int i = 0; i < 3; ++i) out[i] = dir[i] * m;
+    return 0;
+  }
+  case OV_FLOW_COSCOS: /* :324-344 */
+    out[0] = out[1] = 0;
+    out[2] = a[1] * (1 + cos(a[0] * x) * cos(a[0] * y) * exp(-z / a[2]));
+    return 0;
+  case OV_FLOW_COSCOS_DIVFREE: /* :347-370 */
+    out[0] = a[1] / 2 / a[2] / a[0] * sin(a[0] * x) * cos(a[0] * y) * exp(-z / a[2]);
+    out[1] = a[1] / 2 / a[2] / a[0] * cos(a[0] * x) * sin(a[0] * y) * exp(-z / a[2]);
+    out[2] = a[1] * (1 + cos(a[0] * x) * cos(a[0] * y) * exp(-z / a[2]));
+    return 0;
+  case OV_FLOW_COS_DIVFREE: /* :372-394 */
+    out[0] = a[1] / a[2] / a[0] * sin(a[0] * x) * exp(-z / a[2]);
+    out[1] = 0;
+    out[2] = a[1] * (1 + cos(a[0] * x) * exp(-z / a[2]));
+    return 0;
+  case OV_FLOW_SIMPLE_SHEAR: /* :396-413 */
+    out[0] = out[1] = 0;
+    out[2] = a[1] + a[0] * x;
+    return 0;
+  case OV_FLOW_ROTOSC: /* :415-433 */
+  case OV_FLOW_ROTOSC_BOUNDARY: /* the table runs get_v_rotosc for it, :156-157 */
+  case OV_FLOW_DG: { /* :438-457 */
+    /* z x r written out as vec3_cross (vec3_maths.c:304-312) with u = (0,0,1) */
+    const double v[3] = {0.0 * z - 1.0 * y, 1.0 * x - 0.0 * z, 0.0 * y - 0.0 * x};
+    const double m = fl->type == OV_FLOW_DG ? a[0] : a[1] * cos(2 * M_PI * a[0] * t);
+    for (int i = 0; i < 3; ++i) out[i] = v[i] * m;
+    if (fl->type == OV_FLOW_DG) out[2] += a[1];
+    return 0;
+  }
+  case OV_FLOW_CONSTANT_ACCELERATION: /* :483-495 */
+    out[0] = a[0] * t;
+    out[1] = out[2] = 0;
+    return 0;
+  case OV_FLOW_PARABOLIC: /* :497-512 */
+    out[0] = a[1] * z * (a[0] - z);
+    out[1] = out[2] = 0;
+    return 0;
+  default:
+    return OV_E_INVALID;
+  }
+}
+
+int ovo_flow(const ov_flow *fl, const double *r, double t, double *out) {
+  return flow_eval(fl, r, t, out);
 }
 
 /* ---- octree (reference geometry and multipole) --------------------------- */
@@ -472,7 +578,6 @@ void ovo_octree_stats(long *n_cells, long *n_leaf_pairs, long *n_multipoles) {
  * chords to the two neighbours (tangle.c:530-551). */
 static int point_velocity(const ov_tangle_state *t, int k, double time, const ov_params *p,
                           const otnode *tree, double vs[3], double vel[3], long *pairs, long *mps) {
-  (void)time;
   const double *r = t->vnodes[k].p;
   const double *tg = t->tangents[k].p, *nm = t->normals[k].p;
   int f = t->connections[k].forward, rv = t->connections[k].reverse;
@@ -484,7 +589,7 @@ static int point_velocity(const ov_tangle_state *t, int k, double time, const ov
   for (int i = 0; i < 3; ++i) lia[i] = lia[i] * beta;
   for (int i = 0; i < 3; ++i) vs[i] = lia[i];
   double e[3];
-  int rc = flow_eval(&p->vs, e);
+  int rc = flow_eval(&p->vs, r, time, e);
   if (rc) return rc;
   for (int i = 0; i < 3; ++i) vs[i] = vs[i] + e[i];
 
@@ -528,7 +633,7 @@ static int point_velocity(const ov_tangle_state *t, int k, double time, const ov
   }
   if (p->use_mutual_friction) {
     double vn[3], dv[3], a1[3], a2[3];
-    rc = flow_eval(&p->vn, vn);
+    rc = flow_eval(&p->vn, r, time, vn);
     if (rc) return rc;
     for (int i = 0; i < 3; ++i) dv[i] = vn[i] - vs[i];
     cross3(tg, dv, a1);
@@ -541,7 +646,8 @@ static int point_velocity(const ov_tangle_state *t, int k, double time, const ov
 
 static int validate(const ov_tangle_state *t, const ov_params *p) {
   if (!t || !p || t->N < 0) return OV_E_INVALID;
-  if (p->vs.type > OV_FLOW_SIMPLE || p->vn.type > OV_FLOW_SIMPLE) return OV_E_UNSUPPORTED;
+  if (p->vs.type < 0 || p->vs.type >= OV_FLOW_COUNT || p->vn.type < 0 || p->vn.type >= OV_FLOW_COUNT)
+    return OV_E_INVALID;
   for (int k = 0; k < t->N; ++k) {
     if (t->status[k].status == OV_EMPTY) {
       if (t->connections[k].forward != -1) return OV_E_INVALID;

@@ -52,6 +52,10 @@ void ovo_octree_stats(long *n_cells, long *n_leaf_pairs, long *n_multipoles);
 long ovo_reconnection_candidates(const ov_tangle_state *t, double rec_dist, double rec_angle,
                                  int *pairs, long max_pairs);
 
+/* The imposed flow fl at position r[3], time t (external_velocity.c:85-512);
+ * OV_E_INVALID for an unknown type. */
+int ovo_flow(const ov_flow *fl, const double *r, double t, double *out);
+
 /* Threads used by the OpenMP loops (0: runtime default). */
 void ovo_set_threads(int n);
 

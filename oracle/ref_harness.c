@@ -36,17 +36,45 @@ static int find_conf(const char *name) {
   return -1;
 }
 
+/* Table names in the order of the OV_FLOW_* codes. */
+static const char *const flow_names[OV_FLOW_COUNT] = {
+    "no flow", "simple", "spherical", "spherical and simple", "cylindrical", "oscillating",
+    "oscillating slit", "coscos", "coscos-divfree", "cos-divfree", "simple-shear", "rotosc",
+    "DG", "rotosc-boundary", "constant acceleration", "parabolic"};
+
+/* The table entry of f's type with f's parameter values filled in by name:
+ * "external_v" <- v, "wave" <- w, "polarization" <- q, the scalars <- a[] in
+ * table order (entries listed past n_params included, as rotosc-boundary's
+ * delta). */
 static int set_flow(struct v_conf_t *dst, const ov_flow *f) {
-  if (f->type == OV_FLOW_NONE) {
-    *dst = v_confs[find_conf("no flow")];
-    return 0;
+  if (f->type < 0 || f->type >= OV_FLOW_COUNT) return -1;
+  const int k = find_conf(flow_names[f->type]);
+  if (k < 0) return -1;
+  *dst = v_confs[k];
+  int scalar = 0;
+  for (int j = 0; j < 32 && dst->v_params[j].name; ++j) {
+    struct v_param_t *q = &dst->v_params[j];
+    if (strcmp(q->name, "external_v") == 0)
+      q->value.vector = vec3(f->v[0], f->v[1], f->v[2]);
+    else if (strcmp(q->name, "wave") == 0)
+      q->value.vector = vec3(f->w[0], f->w[1], f->w[2]);
+    else if (strcmp(q->name, "polarization") == 0)
+      q->value.vector = vec3(f->q[0], f->q[1], f->q[2]);
+    else if (scalar < 4)
+      q->value.scalar = f->a[scalar++];
   }
-  if (f->type == OV_FLOW_SIMPLE) {
-    *dst = v_confs[find_conf("simple")];
-    dst->v_params[0].value.vector = vec3(f->v[0], f->v[1], f->v[2]);
-    return 0;
-  }
-  return -1;
+  return 0;
+}
+
+/* The reference's flow function of f at r, t (its return value ignored, as
+ * get_vn/get_vs do). */
+int ovref_flow(const ov_flow *f, const double *r, double t, double *out) {
+  struct v_conf_t c;
+  if (set_flow(&c, f)) return -1;
+  struct vec3d w = vec3(r[0], r[1], r[2]), res = vec3(0, 0, 0);
+  c.fun(&w, t, &res, &c);
+  for (int i = 0; i < 3; ++i) out[i] = res.p[i];
+  return 0;
 }
 
 /* Apply an ov_params to the reference globals. */

@@ -44,7 +44,7 @@ def test_params_default_matches_python():
     for name, _ in T.Params._fields_:
         a, b = getattr(p, name), getattr(q, name)
         if isinstance(a, T.Flow):
-            assert a.type == b.type
+            assert a.type == b.type == T.FLOW_NONE
         else:
             assert a == b, name
 
@@ -57,6 +57,8 @@ LAYOUT_C = r"""
 int main(void) {
   printf("ov_tangle_state size %zu\n", sizeof(ov_tangle_state));
   printf("ov_params size %zu\n", sizeof(ov_params));
+  printf("ov_flow size %zu\n", sizeof(ov_flow));
+  F(ov_flow, v) F(ov_flow, a) F(ov_flow, w) F(ov_flow, q)
   F(ov_tangle_state, connections) F(ov_tangle_state, status) F(ov_tangle_state, box)
   F(ov_tangle_state, bimg) F(ov_tangle_state, N) F(ov_tangle_state, total_free)
   F(ov_params, max_curvature_scale) F(ov_params, BH_quadtree) F(ov_params, precision)
@@ -79,10 +81,11 @@ def test_ctypes_layout_matches_c():
         vals[(a, b)] = int(c)
     assert vals[("ov_tangle_state", "size")] == C.sizeof(T.TangleState)
     assert vals[("ov_params", "size")] == C.sizeof(T.Params)
+    assert vals[("ov_flow", "size")] == C.sizeof(T.Flow)
     for (struct, field), off in vals.items():
         if field == "size":
             continue
-        cls = T.TangleState if struct == "ov_tangle_state" else T.Params
+        cls = {"ov_tangle_state": T.TangleState, "ov_params": T.Params, "ov_flow": T.Flow}[struct]
         assert getattr(cls, field).offset == off, (struct, field)
 
 
