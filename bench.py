@@ -182,17 +182,35 @@ def adjoint_section(ctx, T):
     return res
 
 
+def host_threads():
+    """Threads the CPU baseline runs on: OMP_NUM_THREADS when set (the box
+    gives each GPU a share of its cores and sets it), else every core this
+    process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def main():
+    t_start = time.time()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=20, help="RK4 steps to time (clamped by --budget)")
+    ap.add_argument("--warmup", type=int, default=1,
+                    help="untimed RK4 steps of the workload before the timed ones (at most 1 is run)")
+    ap.add_argument("--budget", type=float, default=450.0,
+                    help="wall seconds for the whole run: the timed steps stop when the next one "
+                         "would not fit, the optional sections run only while time is left")
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--dt", type=float, default=1e-6)
     ap.add_argument("--near-radius", type=float, default=0.02)
-    ap.add_argument("--fp64", type=int, default=1, help="FP64 evaluations for the comparison (0: skip)")
-    ap.add_argument("--cpu-sample", type=int, default=64, help="targets of the CPU baseline (0: skip)")
+    ap.add_argument("--fp64", type=int, default=1, help="FP64 evaluation for the comparison (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=256, help="targets of the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--ensemble", type=int, default=256,
                     help="tangles of the batched-ensemble measurement (0: skip)")
@@ -224,41 +242,83 @@ def main():
               "of positions per RK stage)", flush=True)
         sys.exit(2)
 
+    def left():
+        return args.budget - (time.time() - t_start)
+
     from openvort_amd import tangle as T
     from openvort_amd.gpu import Context
 
-    t0 = time.time()
     tg = T.bh_scale(seed=args.seed, n_points=args.points)
     n = len(tg.active)
-    log(f"workload: {n} points, generated in {time.time() - t0:.1f}s")
+    log(f"workload: {n} points ({time.time() - t_start:.1f}s)")
     common = dict(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, hyperfriction=0)
     p_mx = T.default_params(precision=T.PREC_MIXED, near_radius=args.near_radius, **common)
     p_64 = T.default_params(precision=T.PREC_FP64, **common)
 
     ctx = Context(0)
-    # load the code objects on a tiny case (not a step of the workload; local
-    # induction only, so the profile of the direct-sum kernels holds only
-    # launches on the workload)
-    ctx.velocities(T.ring(64), 0.0, T.default_params(LIA_only=1, **common))
+    # load every code object of the step on a tiny tangle (not a step of the
+    # workload, not profiled)
+    for pp in (p_mx, p_64):
+        ctx.rk4_run(T.ring(128), 0.0, 1e-6, 1, pp)
 
-    for w in range(args.warmup):
-        x = tg.copy()
-        ctx.rk4_run(x, 0.0, args.dt, 1, p_mx)
-        log(f"warmup step {w}: {ctx.last_device_ms():.1f} ms")
-
-    # ---- timed: K RK4 steps, device-resident -------------------------------
-    x = tg.copy()
+    out = {}
+    # ---- one mixed and one FP64 evaluation of the workload's positions ------
+    a = tg.copy()
     ctx.reset_stats()
-    wall0 = time.time()
-    ctx.rk4_run(x, 0.0, args.dt, args.steps, p_mx, profile=True)
-    wall = time.time() - wall0
-    dev_ms = ctx.last_device_ms()
+    ctx.velocities(a, 0.0, p_mx, profile=True)
+    mx_ms = ctx.last_device_ms()
     I_eval = ctx.last_interactions()
+    out["mixed_eval_ms"] = round(mx_ms, 2)
+    out["mixed_interactions_per_s_eval"] = I_eval / (mx_ms / 1e3)
+    log(f"mixed evaluation: {mx_ms:.1f} ms ({time.time() - t_start:.0f}s)")
+    b = None
+    if args.fp64 > 0:
+        b = tg.copy()
+        ctx.reset_stats()
+        ctx.velocities(b, 0.0, p_64, profile=True)
+        f64_ms = ctx.last_device_ms()
+        nl64, kd64 = ctx.kernel_stats("k_direct_fp64")
+        act = tg.active
+        out["fp64_eval_ms"] = round(f64_ms, 2)
+        out["fp64_interactions_per_s"] = I_eval / (f64_ms / 1e3)
+        out["mixed_vs_fp64"] = round(f64_ms / mx_ms, 3)
+        out["err_vels_mixed_vs_fp64"] = err_stats(a.vels[act], b.vels[act])
+        out["err_vs_mixed_vs_fp64"] = err_stats(a.vs[act], b.vs[act])
+        fp64_ach = FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3)
+        out["fp64_roofline"] = {"kernel": "k_direct_fp64", "bound": "valu",
+                                "achieved": round(fp64_ach / 1e12, 3), "peak": PEAK_FP64 / 1e12,
+                                "unit": "TFLOP/s", "frac": round(fp64_ach / PEAK_FP64, 4)}
+        log(f"fp64 evaluation {f64_ms:.1f} ms, err {out['err_vels_mixed_vs_fp64']} "
+            f"({time.time() - t_start:.0f}s)")
+
+    # ---- RK4 steps: at most one warm-up step, then timed steps while the
+    # budget holds (a step is about four evaluations) -------------------------
+    est_step_s = 4.2 * mx_ms / 1e3 + 0.5
+    reserve_s = 40.0 + (30.0 if args.cpu_sample > 0 else 0.0)   # CPU leg + print
+    warm = 0
+    x = tg.copy()
+    t_sim = 0.0
+    if args.warmup > 0 and left() - reserve_s > 2 * est_step_s:
+        ctx.rk4_run(x, t_sim, args.dt, 1, p_mx)
+        t_sim += args.dt
+        warm = 1
+        est_step_s = ctx.last_device_ms() / 1e3 + 0.5
+        log(f"warm-up step: {ctx.last_device_ms():.1f} ms ({time.time() - t_start:.0f}s)")
+    ctx.reset_stats()
+    step_ms = []
+    while len(step_ms) < max(args.steps, 1):
+        if step_ms and left() - reserve_s < est_step_s:
+            break
+        ctx.rk4_run(x, t_sim, args.dt, 1, p_mx, profile=True)
+        t_sim += args.dt
+        step_ms.append(ctx.last_device_ms())
+        est_step_s = max(step_ms) / 1e3 + 0.5
+        log(f"timed step {len(step_ms)}: {step_ms[-1]:.1f} ms ({time.time() - t_start:.0f}s)")
+    steps = len(step_ms)
+    dev_ms = float(sum(step_ms))
+    ms_per_step = dev_ms / steps
+    value = 4 * I_eval * steps / (dev_ms / 1e3)
     n_launch, kd_ms = ctx.kernel_stats("k_direct_mixed")
-    ms_per_step = dev_ms / args.steps
-    value = 4 * I_eval * args.steps / (dev_ms / 1e3)
-    log(f"timed: {args.steps} rk4 steps, device {dev_ms:.1f} ms, wall {wall:.1f} s, "
-        f"{value:.4g} int/s, k_direct {n_launch} launches {kd_ms:.1f} ms")
     kd_avg_s = kd_ms / max(n_launch, 1) / 1e3
     achieved = FLOP_PER_INTERACTION * I_eval / kd_avg_s
     kernels = {}
@@ -268,36 +328,76 @@ def main():
         if nl:
             kernels[k] = {"launches": nl, "avg_ms": round(ms / nl, 4)}
 
-    # ---- one mixed and one FP64 evaluation of the same positions ------------
-    out = {}
-    if args.fp64 > 0:
-        a = tg.copy()
-        ctx.reset_stats()
-        ctx.velocities(a, 0.0, p_mx, profile=True)
-        mx_ms = ctx.last_device_ms()
-        b = tg.copy()
-        ctx.reset_stats()
-        ctx.velocities(b, 0.0, p_64, profile=True)
-        f64_ms = ctx.last_device_ms()
-        nl64, kd64 = ctx.kernel_stats("k_direct_fp64")
-        act = tg.active
-        out["fp64_eval_ms"] = round(f64_ms, 2)
-        out["mixed_eval_ms"] = round(mx_ms, 2)
-        out["fp64_interactions_per_s"] = I_eval / (f64_ms / 1e3)
-        out["mixed_interactions_per_s_eval"] = I_eval / (mx_ms / 1e3)
-        out["mixed_vs_fp64"] = round(f64_ms / mx_ms, 3)
-        out["err_vels_mixed_vs_fp64"] = err_stats(a.vels[act], b.vels[act])
-        out["err_vs_mixed_vs_fp64"] = err_stats(a.vs[act], b.vs[act])
-        out["fp64_roofline"] = {
-            "kernel": "k_direct_fp64", "bound": "valu", "achieved": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / 1e12, 3),
-            "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-            "frac": round(FLOP_PER_INTERACTION * I_eval / (kd64 / nl64 / 1e3) / PEAK_FP64, 4)}
-        log(f"fp64 eval {f64_ms:.1f} ms, mixed eval {mx_ms:.1f} ms, "
-            f"err {out['err_vels_mixed_vs_fp64']}")
+    # ---- CPU baseline: the reference's update_velocity on sampled targets ---
+    cpu = None
+    c = None
+    eng = None
+    kind = None
+    threads = args.cpu_threads or host_threads()
+    if args.cpu_sample > 0:
+        from oracle.oracle import Oracle, Reference
+        rng = np.random.default_rng(2002)
+        idx = np.sort(rng.choice(tg.active, size=min(args.cpu_sample, n), replace=False)).astype(np.int32)
+        c = tg.copy()
+        Oracle(threads).update_tangents_normals(c)
+        if Reference.available():
+            kind, eng = "reference", Reference(threads)
+        else:
+            kind, eng = "port", Oracle(threads)
+        c0 = time.time()
+        vs_c, vel_c = eng.velocities_sample(c, 0.0, p_64, idx)
+        cpu_s = time.time() - c0
+        inter = len(idx) * (27 * n - 2)
+        cpu = {"value": inter / cpu_s, "unit": "seg-interactions/s", "cores": threads, "kind": kind,
+               "nproc": os.cpu_count(),
+               "sample": f"{len(idx)} seeded targets of the same tangle (seed 2002), FP64 direct sum "
+                         f"+ 26 images, reference update_velocity per target ({inter:.3g} "
+                         f"interactions, {cpu_s:.1f} s)"}
+        log(f"cpu baseline ({kind}, {threads} threads): {cpu['value']:.4g} int/s in {cpu_s:.1f}s")
+        if b is not None:
+            out["err_vels_fp64_vs_cpu_sample"] = err_stats(b.vels[idx], vel_c)
+        out["err_vels_mixed_vs_cpu_sample"] = err_stats(a.vels[idx], vel_c)
 
-    # ---- Barnes-Hut octree vs the FP64 direct sum (seeded 4096-target sample)
+    res = {
+        "metric": "Biot-Savart seg-interactions/s & RK4 steps/s, mixed vs FP64, 1M pts @1e-6 rel",
+        "value": value,
+        "unit": "seg-interactions/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "steps_requested": args.steps,
+        "warmup": warm,
+        "warmup_requested": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step_median": round(float(np.median(step_ms)), 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "Barnes-Hut-scale periodic tangle: box 0.2 cm, 26 images, 64 perturbed "
+                               "box-spanning lines + log-uniform rings (r 0.002-0.03 cm), spacing "
+                               "1e-4 cm, seed %d" % args.seed,
+                   "points": n, "images": 26, "method": "direct", "precision": "mixed",
+                   "near_radius_cm": args.near_radius, "dt_s": args.dt,
+                   "interactions_per_eval": I_eval},
+        "rk4_steps_per_s": 1e3 / ms_per_step,
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
+                     "traffic": measured_traffic(n, args.seed, args.near_radius),
+                     "kernel": "k_direct_mixed", "avg_launch_ms": round(kd_avg_s * 1e3, 2),
+                     "launches": n_launch, "flop_per_interaction": FLOP_PER_INTERACTION},
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    res.update(out)
+    print(json.dumps(res), flush=True)
+    log(f"headline printed ({time.time() - t_start:.0f}s)")
+
+    # ---- optional sections, each only while the budget allows it ----------
+    skipped = []
     tree_samples = {}   # theta -> (octree entry, GPU FP64 reference-multipole vels on the sample)
-    if args.tree > 0:
+    samp = None
+    if args.tree > 0 and left() > 60:
         rng_t = np.random.default_rng(2002)
         samp = np.sort(rng_t.choice(tg.active, size=min(4096, n), replace=False))
         tree = []
@@ -306,11 +406,11 @@ def main():
         for prec, pname, mp in ((T.PREC_FP64, "fp64", T.MP_REFERENCE), (T.PREC_MIXED, "mixed", T.MP_REFERENCE),
                                 (T.PREC_FP64, "fp64", T.MP_DIPOLE)):
             for th in (0.3, 0.5, 0.7):
-                x = tg.copy()
+                xt = tg.copy()
                 ctx.reset_stats()
                 ctx.tree_stats()
-                ctx.velocities(x, 0.0, T.default_params(precision=prec, use_BH=1, BH_resolution=th,
-                                                        multipole=mp, **common), profile=True)
+                ctx.velocities(xt, 0.0, T.default_params(precision=prec, use_BH=1, BH_resolution=th,
+                                                         multipole=mp, **common), profile=True)
                 ms = ctx.last_device_ms()
                 cells, pairs, mps = ctx.tree_stats()
                 kname = "k_octree_fp64" if prec == T.PREC_FP64 else "k_octree_mixed"
@@ -320,10 +420,10 @@ def main():
                              "traversal_ms": round(ctx.kernel_stats(kname)[1], 2),
                              "cells": int(cells), "leaf_pairs": pairs, "multipoles": mps,
                              "build_kernels_ms": {kn: round(ctx.kernel_stats(kn)[1], 3) for kn in OCT_BUILD}})
-                if args.fp64 > 0:
-                    tree[-1]["err_vels_vs_fp64_direct_sample"] = err_stats(x.vels[samp], b.vels[samp])
+                if b is not None:
+                    tree[-1]["err_vels_vs_fp64_direct_sample"] = err_stats(xt.vels[samp], b.vels[samp])
                 if mp == T.MP_REFERENCE and prec == T.PREC_FP64:
-                    tree_samples[th] = (len(tree) - 1, x.vels[samp].copy())
+                    tree_samples[th] = (len(tree) - 1, xt.vels[samp].copy())
                 log(f"tree {pname} {tree[-1]['multipole']} theta {th}: {ms:.1f} ms, "
                     f"err {tree[-1].get('err_vels_vs_fp64_direct_sample')}")
         hbm = tree_hbm_bytes()
@@ -334,16 +434,32 @@ def main():
                      "GB_per_s": round(hbm[kn] / (build[kn] / 1e3) / 1e9, 1) if build.get(kn) else None}
                 for kn in OCT_BUILD if kn in hbm}
         out["octree"] = tree
+    elif args.tree > 0:
+        skipped.append("octree")
 
-    # ---- reconnection candidates (reference reconnect's pair loop) ----------
-    if args.reconnect > 0:
-        x = tg.copy()
-        ctx.update_tangents_normals(x)
+    # the reference's own octree on the octree section's 4096-target sample
+    # (build + evaluation per theta), against the GPU tree's velocities
+    if tree_samples and kind == "reference" and left() > 60:
+        for th, (entry, vel_g) in tree_samples.items():
+            pr = T.default_params(precision=T.PREC_FP64, use_BH=1, BH_resolution=th, **common)
+            c0 = time.time()
+            _, vel_r = eng.velocities_sample(c, 0.0, pr, samp.astype(np.int32))
+            out["octree"][entry]["cpu_reference"] = {
+                "threads": threads, "targets": int(len(samp)),
+                "build_plus_eval_s": round(time.time() - c0, 2),
+                "err_vels_gpu_vs_reference_tree": err_stats(vel_g, vel_r)}
+            log(f"cpu reference octree theta {th}: {out['octree'][entry]['cpu_reference']}")
+    elif tree_samples and kind == "reference":
+        skipped.append("octree_cpu_reference")
+
+    if args.reconnect > 0 and left() > 30:
+        xr = tg.copy()
+        ctx.update_tangents_normals(xr)
         rec = []
         for rd in (1e-3, 2e-4):
-            ctx.reconnection_candidates(x, rd, 0.087)          # sizes the buffers
+            ctx.reconnection_candidates(xr, rd, 0.087)          # sizes the buffers
             w0 = time.time()
-            cand = ctx.reconnection_candidates(x, rd, 0.087)
+            cand = ctx.reconnection_candidates(xr, rd, 0.087)
             wall_ms = (time.time() - w0) * 1e3
             rec.append({"rec_dist_cm": rd, "rec_angle": 0.087, "candidates": int(len(cand)),
                         "device_ms": round(ctx.last_device_ms(), 2), "wall_ms": round(wall_ms, 2),
@@ -351,115 +467,65 @@ def main():
             log(f"reconnection candidates rec_dist {rd}: {len(cand)} pairs, "
                 f"{rec[-1]['device_ms']} ms device, {wall_ms:.1f} ms wall")
         out["reconnection_candidates"] = rec
+    elif args.reconnect > 0:
+        skipped.append("reconnection_candidates")
 
-    # ---- ensemble: independent tangles, one launch per stage vs a loop -------
-    if args.ensemble > 0:
-        ens = T.ensemble(n_tangles=args.ensemble)
-
-        def ep(kw):
-            kw = dict(kw)
-            kw.pop("temperature")
-            return T.default_params(vortex_width=1.3e-8, hyperfriction=0, precision=T.PREC_MIXED,
-                                    near_radius=args.near_radius, **kw)
-        ps = [ep(kw) for _, kw in ens]
-        ts = [t.copy() for t, _ in ens]
-        ctx.rk4_run_batch(ts, 0.0, 5e-6, 0, ps)          # layout + tuner, no step
-        ts = [t.copy() for t, _ in ens]
-        ctx.rk4_run_batch(ts, 0.0, 5e-6, args.ensemble_steps, ps)
-        b_ms = ctx.last_device_ms()
-        U, blocks, tune_ms = ctx.batch_tuning()
-        loop_ms = 0.0
-        ts2 = [t.copy() for t, _ in ens]
-        for x, pp in zip(ts2, ps):
-            ctx.rk4_run(x, 0.0, 5e-6, args.ensemble_steps, pp)
-            loop_ms += ctx.last_device_ms()
-        npts = sum(len(t.active) for t, _ in ens)
-        inter = sum(27.0 * len(t.active) ** 2 for t, _ in ens)
-        out["ensemble"] = {
-            "tangles": args.ensemble, "points": npts, "steps": args.ensemble_steps,
-            "batched_ms_per_step": round(b_ms / args.ensemble_steps, 2),
-            "loop_ms_per_step": round(loop_ms / args.ensemble_steps, 2),
-            "aggregate_steps_per_s_batched": args.ensemble * args.ensemble_steps / (b_ms / 1e3),
-            "aggregate_steps_per_s_loop": args.ensemble * args.ensemble_steps / (loop_ms / 1e3),
-            "interactions_per_s_batched": 4 * inter * args.ensemble_steps / (b_ms / 1e3),
-            "tuned_unit_items": U, "tuned_blocks": blocks, "tuning_ms": round(tune_ms, 1),
-            "max_position_diff_batched_vs_loop": float(max(np.abs(a.vnodes - c.vnodes).max()
-                                                         for a, c in zip(ts, ts2)))}
+    if args.ensemble > 0 and left() > 60:
+        out["ensemble"] = ensemble_section(ctx, T, args)
         log(f"ensemble: {out['ensemble']}")
+    elif args.ensemble > 0:
+        skipped.append("ensemble")
 
-    # ---- adjoint: Kelvin-ring loss gradient through 200 RK4 steps ----------
-    if args.adjoint > 0:
+    if args.adjoint > 0 and left() > 90:
         out["adjoint"] = adjoint_section(ctx, T)
         log(f"adjoint: {out['adjoint']}")
-
-    # ---- CPU baseline: the reference's update_velocity on sampled targets ---
-    cpu = None
-    if args.cpu_sample > 0:
-        from oracle.oracle import Oracle, Reference
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rng = np.random.default_rng(2002)
-        idx = np.sort(rng.choice(tg.active, size=args.cpu_sample, replace=False)).astype(np.int32)
-        c = tg.copy()
-        Oracle(threads).update_tangents_normals(c)
-        if Reference.available():
-            kind, eng = "reference", Reference(threads)
-        else:
-            kind, eng = "port", Oracle(threads)
-        c0 = time.time()
-        vs_c, vel_c = eng.velocities_sample(c, 0.0, p_64, idx)
-        cpu_s = time.time() - c0
-        n_seg = n
-        inter = args.cpu_sample * (27 * n_seg - 2)
-        cpu = {"value": inter / cpu_s, "unit": "seg-interactions/s", "cores": threads, "kind": kind,
-               "sample": f"{args.cpu_sample} seeded targets of the same tangle, FP64 direct sum + "
-                         f"26 images ({inter:.3g} interactions, {cpu_s:.1f} s)"}
-        log(f"cpu baseline ({kind}, {threads} threads): {cpu['value']:.4g} int/s in {cpu_s:.1f}s")
-        if args.fp64 > 0:
-            out["err_vels_fp64_vs_cpu_sample"] = err_stats(b.vels[idx], vel_c)
-            out["err_vels_mixed_vs_cpu_sample"] = err_stats(a.vels[idx], vel_c)
-        # the reference's own octree on the octree section's 4096-target sample
-        # (build + evaluation per theta), against the GPU tree's velocities
-        if tree_samples and kind == "reference":
-            for th, (entry, vel_g) in tree_samples.items():
-                pr = T.default_params(precision=T.PREC_FP64, use_BH=1, BH_resolution=th, **common)
-                c0 = time.time()
-                _, vel_r = eng.velocities_sample(c, 0.0, pr, samp.astype(np.int32))
-                out["octree"][entry]["cpu_reference"] = {
-                    "threads": threads, "targets": int(len(samp)),
-                    "build_plus_eval_s": round(time.time() - c0, 2),
-                    "err_vels_gpu_vs_reference_tree": err_stats(vel_g, vel_r)}
-                log(f"cpu reference octree theta {th}: {out['octree'][entry]['cpu_reference']}")
+    elif args.adjoint > 0:
+        skipped.append("adjoint")
 
     ctx.close()
-    res = {
-        "metric": "Biot-Savart seg-interactions/s (mixed FP32/FP64 direct sum, RK4 steps)",
-        "value": value,
-        "unit": "seg-interactions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": "periodic tangle, box 0.2 cm, 64 perturbed lines + log-uniform "
-                               "rings, spacing 1e-4 cm, seed %d" % args.seed,
-                   "points": n, "images": 26, "method": "direct", "precision": "mixed",
-                   "near_radius_cm": args.near_radius, "dt_s": args.dt,
-                   "interactions_per_eval": I_eval},
-        "rk4_steps_per_s": 1e3 / ms_per_step,
-        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
-                     "traffic": measured_traffic(n, args.seed, args.near_radius),
-                     "kernel": "k_direct_mixed", "avg_launch_ms": round(kd_avg_s * 1e3, 2),
-                     "flop_per_interaction": FLOP_PER_INTERACTION},
-        "cpu_baseline": cpu,
-        "kernels": kernels,
-    }
+    out["sections_skipped_for_budget"] = skipped
+    out["wall_s"] = round(time.time() - t_start, 1)
     res.update(out)
     print(json.dumps(res), flush=True)
+
+
+def ensemble_section(ctx, T, args):
+    """The 256-tangle ensemble (box 0.05 cm, 500-20000 points each, own
+    counterflow and temperature per tangle): batched RK4 steps, one launch per
+    evaluation for the whole ensemble, against a per-tangle loop of
+    ov_gpu_rk4_run on the same tangles."""
+    ens = T.ensemble(n_tangles=args.ensemble)
+
+    def ep(kw):
+        kw = dict(kw)
+        kw.pop("temperature")
+        return T.default_params(vortex_width=1.3e-8, hyperfriction=0, precision=T.PREC_MIXED,
+                                near_radius=args.near_radius, **kw)
+    ps = [ep(kw) for _, kw in ens]
+    ts = [t.copy() for t, _ in ens]
+    ctx.rk4_run_batch(ts, 0.0, 5e-6, 0, ps)          # layout + tuner, no step
+    ts = [t.copy() for t, _ in ens]
+    ctx.rk4_run_batch(ts, 0.0, 5e-6, args.ensemble_steps, ps)
+    b_ms = ctx.last_device_ms()
+    U, blocks, tune_ms = ctx.batch_tuning()
+    loop_ms = 0.0
+    ts2 = [t.copy() for t, _ in ens]
+    for x, pp in zip(ts2, ps):
+        ctx.rk4_run(x, 0.0, 5e-6, args.ensemble_steps, pp)
+        loop_ms += ctx.last_device_ms()
+    npts = sum(len(t.active) for t, _ in ens)
+    inter = sum(27.0 * len(t.active) ** 2 for t, _ in ens)
+    return {
+        "tangles": args.ensemble, "points": npts, "steps": args.ensemble_steps,
+        "batched_ms_per_step": round(b_ms / args.ensemble_steps, 2),
+        "loop_ms_per_step": round(loop_ms / args.ensemble_steps, 2),
+        "batched_vs_loop": round(loop_ms / b_ms, 3),
+        "aggregate_steps_per_s_batched": args.ensemble * args.ensemble_steps / (b_ms / 1e3),
+        "aggregate_steps_per_s_loop": args.ensemble * args.ensemble_steps / (loop_ms / 1e3),
+        "interactions_per_s_batched": 4 * inter * args.ensemble_steps / (b_ms / 1e3),
+        "tuned_unit_items": U, "tuned_blocks": blocks, "tuning_ms": round(tune_ms, 1),
+        "max_position_diff_batched_vs_loop": float(max(np.abs(a.vnodes - c.vnodes).max()
+                                                     for a, c in zip(ts, ts2)))}
 
 
 if __name__ == "__main__":

@@ -651,7 +651,7 @@ int build_tree(ov_ctx* ctx, const double* X) {
   HIPCHK(hipMemsetAsync(ctx->t_maxd, 0, sizeof(int), ctx->stream));
   if ((rc = launch(ctx, "k_oct_moments", k_oct_moments, dim3((unsigned)((ncell + 3) / 4)), dim3(256),
                    (const OctCell*)ctx->t_cells, ncell, (const LeafSeg*)ctx->t_segs, ctx->t_mom,
-                   ctx->t_big, ctx->t_maxd)))
+                   ctx->t_big, ctx->t_maxd, ctx->op.dipole)))
     return rc;
   int n_big = 0;
   HIPCHK(hipMemcpyAsync(&n_big, ctx->t_maxd, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -668,7 +668,7 @@ int build_tree(ov_ctx* ctx, const double* X) {
                      ctx->t_big_part)) ||
         (rc = launch(ctx, "k_oct_big_sum2", k_oct_big_sum2, gb, dim3(256), (const OctCell*)ctx->t_cells,
                      (const int*)ctx->t_big, (const int*)ctx->t_maxd, (const LeafSeg*)ctx->t_segs,
-                     ctx->t_big_part)) ||
+                     ctx->t_big_part, ctx->op.dipole)) ||
         (rc = launch(ctx, "k_oct_big_fin", k_oct_big_fin, dim3((unsigned)((n_big + 63) / 64)), dim3(64),
                      (const OctCell*)ctx->t_cells, (const int*)ctx->t_big, (const int*)ctx->t_maxd,
                      (const double*)ctx->t_big_part, ctx->t_mom)))

@@ -1020,10 +1020,12 @@ __global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ P,
 // RK4 stages (reference vortex_dynamics.c:36-52, 58-117)
 // ---------------------------------------------------------------------------
 
-// Xs = X0 + h K (euler_step2: move = v dt, then x + move)
+// Xs = X0 + h K (euler_step2: move = v dt, then x + move, each rounded as the
+// reference rounds it: no fused multiply-add)
 __global__ void __launch_bounds__(256) k_stage(const double* __restrict__ X0,
                                                const double* __restrict__ K, double h, int n,
                                                long cap, double* __restrict__ Xs) {
+#pragma clang fp contract(off)
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
 #pragma unroll
@@ -1033,13 +1035,15 @@ __global__ void __launch_bounds__(256) k_stage(const double* __restrict__ X0,
   }
 }
 
-// X0 += dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4, accumulated in that order.
+// X0 += dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4, accumulated in that order
+// with every product and sum rounded separately (vortex_dynamics.c:99-110).
 __global__ void __launch_bounds__(256) k_combine(double* __restrict__ X0,
                                                  const double* __restrict__ K1,
                                                  const double* __restrict__ K2,
                                                  const double* __restrict__ K3,
                                                  const double* __restrict__ K4, double dt, int n,
                                                  long cap, int* __restrict__ flag) {
+#pragma clang fp contract(off)
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   bool bad = false;

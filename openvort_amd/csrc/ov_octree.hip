@@ -282,7 +282,10 @@ __device__ inline double wave_sum(double v) {
 // 4. Moments of every cell over its points (octree.c:237-266), one
 // wavefront per cell: lane sums over a stride of 64, then a fixed butterfly
 // (deterministic; the order differs from the reference's sequential sum by
-// rounding only).
+// rounding only).  M_ij = sum ds_i (r - centre)_j with r the segment's start
+// (the reference) or, for the dipole expansion (midpoint != 0), its midpoint:
+// the first-order term of a straight segment's field is taken about the
+// midpoint, and start points would leave a first-order error.
 // Cells of more than OCT_BIG points are only listed here (big, n_big) and
 // summed by the k_oct_big_* kernels, OCT_SLICES blocks per cell.
 constexpr int OCT_BIG = 2048;
@@ -291,7 +294,7 @@ constexpr int OCT_SLICES = 64;
 __global__ void __launch_bounds__(256) k_oct_moments(const OctCell* __restrict__ cells, int ncell,
                                                      const LeafSeg* __restrict__ segs,
                                                      double* __restrict__ mom, int* __restrict__ big,
-                                                     int* __restrict__ n_big) {
+                                                     int* __restrict__ n_big, int midpoint) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncell) return;
@@ -315,6 +318,8 @@ __global__ void __launch_bounds__(256) k_oct_moments(const OctCell* __restrict__
       sl[a] = q.a[a] - com[a];
       g[a] += ds[a];
     }
+    if (midpoint)
+      for (int a = 0; a < 3; ++a) sl[a] = sl[a] + 0.5 * ds[a];
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) M[3 * a + b] += ds[a] * sl[b];
   }
@@ -375,7 +380,7 @@ __global__ void __launch_bounds__(256) k_oct_big_sum2(const OctCell* __restrict_
                                                       const int* __restrict__ big,
                                                       const int* __restrict__ n_big,
                                                       const LeafSeg* __restrict__ segs,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, int midpoint) {
   __shared__ double lds[4];
   const int y = blockIdx.y;
   if (y >= *n_big) return;
@@ -397,6 +402,8 @@ __global__ void __launch_bounds__(256) k_oct_big_sum2(const OctCell* __restrict_
       ds[a] = q.b[a] - q.a[a];
       sl[a] = q.a[a] - com[a];
     }
+    if (midpoint)
+      for (int a = 0; a < 3; ++a) sl[a] = sl[a] + 0.5 * ds[a];
     for (int a = 0; a < 3; ++a)
       for (int c = 0; c < 3; ++c) M[3 * a + c] += ds[a] * sl[c];
   }

@@ -53,6 +53,8 @@ class Oracle:
                                                         P(C.c_int), C.c_long]
             lib.ovo_reconnection_candidates.restype = C.c_long
             lib.ovo_flow.argtypes = [P(Flow), P(C.c_double), C.c_double, P(C.c_double)]
+            lib.ovo_cell_far_field.argtypes = [P(TangleState), P(C.c_int), C.c_int, P(C.c_double),
+                                               P(Params), P(C.c_double)]
             Oracle._lib = lib
         self.lib = Oracle._lib
         self.lib.ovo_set_threads(threads)
@@ -116,6 +118,17 @@ class Oracle:
         self.lib.ovo_reconnection_candidates(C.byref(ts), rec_dist, rec_angle,
                                              out.ctypes.data_as(P(C.c_int)), n)
         return out[:n]
+
+    def cell_far_field(self, t: Tangle, ids, r, p: Params):
+        """Multipole value at r of one octree cell holding the points ids."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros(3)
+        ts = t.cstruct()
+        self._rc(self.lib.ovo_cell_far_field(C.byref(ts), ids.ctypes.data_as(P(C.c_int)), len(ids),
+                                             r.ctypes.data_as(P(C.c_double)), C.byref(p),
+                                             out.ctypes.data_as(P(C.c_double))))
+        return out
 
     def octree_stats(self):
         a, b, c = C.c_long(), C.c_long(), C.c_long()

@@ -414,8 +414,12 @@ static double box_max_len(const double lo[3], const double hi[3]) {
 
 /* Cell moments (octree.c:237-266): centre = (sum r) * (1/n); circulation
  * Gamma = sum ds; tensor M_ij = sum ds_i (r - centre)_j, with ds the
- * minimum-image segment of each point that has a forward neighbour. */
-static void ot_moments(otnode *c, const ov_tangle_state *t) {
+ * minimum-image segment of each point that has a forward neighbour and r its
+ * start (the reference's form).  The dipole mode takes r at the segment's
+ * midpoint instead: the first-order term of a straight segment's field is
+ * ds x [-(mid - centre)/|R|^3 + 3 ((mid - centre).R) R/|R|^5], so start
+ * points would leave a first-order error of 3/2 sum (ds.R) ds x R/|R|^5. */
+static void ot_moments(otnode *c, const ov_tangle_state *t, int midpoint) {
   double s[3] = {0, 0, 0};
   for (int k = 0; k < c->n; ++k)
     for (int i = 0; i < 3; ++i) s[i] += t->vnodes[c->ids[k]].p[i];
@@ -427,6 +431,8 @@ static void ot_moments(otnode *c, const ov_tangle_state *t) {
     if (!segment_of(t, c->ids[k], a, b)) continue;
     double ds[3], sl[3];
     for (int i = 0; i < 3; ++i) { ds[i] = b[i] - a[i]; sl[i] = a[i] - c->com[i]; }
+    if (midpoint)
+      for (int i = 0; i < 3; ++i) sl[i] = sl[i] + 0.5 * ds[i];
     for (int i = 0; i < 3; ++i) c->gam[i] += ds[i];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) c->M[i][j] += ds[i] * sl[j];
@@ -439,7 +445,7 @@ static void ot_moments(otnode *c, const ov_tangle_state *t) {
  * it strictly; points on shared faces or outside every child are dropped. */
 static void ot_split(otnode *c, const ov_tangle_state *t, const ov_params *p) {
   if (c->n > 0) ot_cells++;
-  ot_moments(c, t);
+  ot_moments(c, t, p->multipole == OV_MP_DIPOLE);
   if (c->n <= 1) return;
   if (box_min_len(c->lo, c->hi) < p->BH_grain * p->dl_max) return;
   const int nch = p->BH_quadtree ? 4 : 8;
@@ -566,6 +572,20 @@ void ovo_octree_stats(long *n_cells, long *n_leaf_pairs, long *n_multipoles) {
   if (n_cells) *n_cells = ot_cells;
   if (n_leaf_pairs) *n_leaf_pairs = ot_leaf_pairs;
   if (n_multipoles) *n_multipoles = ot_multipoles;
+}
+
+/* Far-field value at r of one cell holding the listed points: the moments
+ * and the multipole of the octree path above (the cell's box plays no part). */
+int ovo_cell_far_field(const ov_tangle_state *t, const int *ids, int n, const double *r,
+                       const ov_params *p, double *out) {
+  if (n <= 0 || !ids) return OV_E_INVALID;
+  otnode c;
+  memset(&c, 0, sizeof c);
+  c.ids = (int *)ids;
+  c.n = n;
+  ot_moments(&c, t, p->multipole == OV_MP_DIPOLE);
+  ot_multipole(&c, r, p, out);
+  return OV_OK;
 }
 
 /* ---- per-point velocity -------------------------------------------------- */

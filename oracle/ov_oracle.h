@@ -46,6 +46,11 @@ void ovo_segment_field(const double a[3], const double b[3], const double r[3],
  * non-empty cells, evaluated leaf segment-point pairs, multipoles. */
 void ovo_octree_stats(long *n_cells, long *n_leaf_pairs, long *n_multipoles);
 
+/* Far-field value at r[3] of one octree cell holding the points ids[0..n-1]
+ * (the cell moments and multipole of the octree path, mode p->multipole). */
+int ovo_cell_far_field(const ov_tangle_state *t, const int *ids, int n, const double *r,
+                       const ov_params *p, double *out);
+
 /* Pass-invariant reconnection tests over all pairs k < l (vortex_dynamics.c:
  * 221-265): writes up to max_pairs (k, l) pairs in loop order, returns the
  * total count. */

@@ -259,3 +259,24 @@ def test_rk4_run_graph_replay_matches_eager_steps(ctx, precision):
         ctx.rk4_run(c, 0.0, 2e-6, 6, p)
         assert np.array_equal(c.vnodes, a.vnodes)
         assert np.array_equal(c.vels, b.vels)
+
+
+@pytest.mark.parametrize("variant", [T.RK_REFERENCE, T.RK_CLASSIC])
+def test_rk4_linear_flow_bitwise(ctx, oracle, variant):
+    """LIA only on a straight periodic line in the linear "DG" flow (local
+    induction exactly zero): the stage positions and the combine are each
+    rounded as the reference rounds them (no fused multiply-add), so the GPU
+    positions equal the oracle's bit for bit, and x + iy is multiplied by the
+    variant's amplification polynomial in z = i rot dt."""
+    from tests.test_oracle import _dg_line
+    rot, vz, dt = 5e4, 0.3, 1e-5
+    t = _dg_line()
+    p = T.default_params(use_mutual_friction=0, hyperfriction=0, LIA_only=1, rk_variant=variant,
+                         vs=("DG", {"rot": rot, "vz": vz}))
+    a, b = _both(t, lambda x: oracle.step(x, 0.0, dt, p), lambda x: ctx.rk4_step(x, 0.0, dt, p))
+    np.testing.assert_array_equal(b.vnodes, a.vnodes)
+    z = 1j * rot * dt
+    amp = (1 + z + 5 * z**2 / 12 + z**3 / 8 + z**4 / 48) if variant == T.RK_REFERENCE else \
+        (1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24)
+    w0 = t.vnodes[:, 0] + 1j * t.vnodes[:, 1]
+    np.testing.assert_allclose(b.vnodes[:, 0] + 1j * b.vnodes[:, 1], amp * w0, rtol=1e-14, atol=0)

@@ -295,3 +295,151 @@ def test_octree_theta_zero_is_direct(oracle):
     assert rel_err(b.vels[act], a.vels[act])[0] < 1e-11
     cells, pairs, mps = oracle.octree_stats()
     assert mps == 0 and cells > len(act)
+
+
+# ---- RK variants against their amplification polynomials -----------------------
+
+def _dg_line(x0=0.01, y0=0.02, n=40, L=0.1):
+    """A straight periodic line along z at (x0, y0): its local induction is
+    exactly zero (s' and s'' have no x or y component), so with LIA only, no
+    friction and the "DG" flow v = rot (-y, x, 0) + vz z as the superfluid flow
+    every point moves by v(r) alone, and the line stays straight."""
+    b = T.TangleBuilder()
+    z = (np.arange(n) + 0.37) * (L / n) - L / 2
+    b.add_loop(np.stack([np.full(n, x0), np.full(n, y0), z], 1))
+    return b.build((-L / 2,) * 3, (L / 2,) * 3, (T.WALL_PERIODIC,) * 6, ())
+
+
+@pytest.mark.parametrize("variant", [T.RK_REFERENCE, T.RK_CLASSIC])
+def test_rk4_linear_flow_amplification(oracle, variant):
+    """In the linear flow d(x + iy)/dt = i rot (x + iy) one step multiplies
+    x + iy by the scheme's amplification polynomial in z = i rot dt:
+    1 + z + 5z^2/12 + z^3/8 + z^4/48 for the reference's k4 at x + dt/2 k3
+    (vortex_dynamics.c:89), 1 + z + z^2/2 + z^3/6 + z^4/24 for the classic
+    scheme.  At rot dt = 0.5 the two differ by 2e-2 relative, so a wrong
+    offset, weight or stage fails."""
+    rot, vz, dt = 5e4, 0.3, 1e-5
+    t = _dg_line()
+    p = T.default_params(use_mutual_friction=0, hyperfriction=0, LIA_only=1, rk_variant=variant,
+                         vs=("DG", {"rot": rot, "vz": vz}))
+    w0 = t.vnodes[:, 0] + 1j * t.vnodes[:, 1]
+    z0 = t.vnodes[:, 2].copy()
+    oracle.step(t, 0.0, dt, p)
+    z = 1j * rot * dt
+    if variant == T.RK_REFERENCE:
+        amp = 1 + z + 5 * z**2 / 12 + z**3 / 8 + z**4 / 48
+    else:
+        amp = 1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24
+    w1 = t.vnodes[:, 0] + 1j * t.vnodes[:, 1]
+    np.testing.assert_allclose(w1, amp * w0, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(t.vnodes[:, 2] - z0, vz * dt, rtol=0, atol=2e-17)
+    other = (1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24) if variant == T.RK_REFERENCE else \
+        (1 + z + 5 * z**2 / 12 + z**3 / 8 + z**4 / 48)
+    assert abs(amp - other) / abs(amp) > 1e-2
+
+
+def test_rk4_stage_times(oracle):
+    """A time-dependent linear flow ("rotosc": rot(t) = amp cos(2 pi f t))
+    fixes the stage times t, t + dt/2, t + dt/2, t + dt (vortex_dynamics.c:
+    79-92): the step equals the scheme run in complex scalars with those
+    times, and differs from it at every other placement of the k4 time."""
+    amp_, f, dt, t0 = 4e4, 2e4, 1e-5, 3e-6
+    lam = lambda tt: 1j * amp_ * math.cos(2 * math.pi * f * tt)
+
+    def scheme(w, h4, t4):
+        k1 = lam(t0) * w
+        k2 = lam(t0 + dt / 2) * (w + dt / 2 * k1)
+        k3 = lam(t0 + dt / 2) * (w + dt / 2 * k2)
+        k4 = lam(t4) * (w + h4 * k3)
+        return w + dt / 6 * k1 + dt / 3 * k2 + dt / 3 * k3 + dt / 6 * k4
+
+    for variant, h4 in ((T.RK_REFERENCE, dt / 2), (T.RK_CLASSIC, dt)):
+        t = _dg_line()
+        p = T.default_params(use_mutual_friction=0, hyperfriction=0, LIA_only=1, rk_variant=variant,
+                             vs=("rotosc", {"freq": f, "amp": amp_}))
+        w0 = t.vnodes[:, 0] + 1j * t.vnodes[:, 1]
+        oracle.step(t, t0, dt, p)
+        w1 = t.vnodes[:, 0] + 1j * t.vnodes[:, 1]
+        np.testing.assert_allclose(w1, scheme(w0, h4, t0 + dt), rtol=1e-13, atol=0)
+        wrong = scheme(w0, h4, t0 + dt / 2)
+        assert np.abs(w1 - wrong).max() / np.abs(w1).max() > 1e-4
+
+
+# ---- the dipole multipole against the direct sum it expands -------------------------
+
+def _loop_tangle(points):
+    b = T.TangleBuilder()
+    b.add_loop(points)
+    return b.build((-1, -1, -1), (1, 1, 1))
+
+
+def _direct(oracle, pts, closed, r):
+    n = len(pts)
+    v = np.zeros(3)
+    for k in range(n if closed else n - 1):
+        v += oracle.segment_field(pts[k], pts[(k + 1) % n], r, KAPPA)
+    return v
+
+
+def _far_errors(oracle, t, ids, seg_pts, closed, centre, direction, dists, mode):
+    p = T.default_params(multipole=mode)
+    out = []
+    for D in dists:
+        r = centre + D * direction
+        ff = oracle.cell_far_field(t, ids, r, p)
+        ex = _direct(oracle, seg_pts, closed, r)
+        out.append(np.linalg.norm(ff - ex) / np.linalg.norm(ex))
+    return np.array(out)
+
+
+def test_dipole_multipole_closed_loop_convergence(oracle):
+    """A cell holding one small closed loop (zero circulation): the dipole
+    expansion's error against the direct sum over its segments falls like
+    a/D (the omitted quadrupole relative to the dipole field), halving per
+    doubling of D.  A flipped or transposed eps:M term, a wrong factor on
+    3 (M R) x R or moments about segment starts instead of midpoints leave an
+    error that does not fall with D (the first two of order 1, the last of
+    order pi/N).  The reference's form (octree.c:303-328, quirks kept) is
+    shown not to converge, which the test relies on to tell the forms apart."""
+    a, n = 1e-3, 96
+    ph = 2 * np.pi * np.arange(n) / n
+    e1 = np.array([0.8, 0.6, 0.0])
+    e2 = np.array([-0.36, 0.48, 0.8])
+    e3 = np.cross(e1, e2)
+    c0 = np.array([0.1, -0.05, 0.02])
+    pts = c0 + a * (np.cos(ph)[:, None] * e1 + 0.6 * np.sin(ph)[:, None] * e2
+                    + 0.3 * np.sin(2 * ph)[:, None] * e3 + 0.2 * np.cos(3 * ph)[:, None] * e1)
+    t = _loop_tangle(pts)
+    ids = np.arange(n)
+    centre = pts.mean(axis=0)
+    direction = np.array([0.3, -0.5, 0.81])
+    direction /= np.linalg.norm(direction)
+    dists = a * np.array([8, 16, 32, 64, 128])
+    e = _far_errors(oracle, t, ids, pts, True, centre, direction, dists, T.MP_DIPOLE)
+    ratios = e[1:] / e[:-1]
+    assert np.all((ratios > 0.42) & (ratios < 0.58)), (e, ratios)
+    assert e[-1] < 3e-2
+    e_ref = _far_errors(oracle, t, ids, pts, True, centre, direction, dists, T.MP_REFERENCE)
+    assert e_ref.min() > 0.1, e_ref
+
+
+def test_dipole_multipole_open_chain_convergence(oracle):
+    """A cell holding an open piece of filament (non-zero circulation Gamma):
+    monopole plus exact first-order term, so the error falls like (L/D)^2,
+    a quarter per doubling of D; a wrong sign of Gamma x R or of either
+    first-order term, or start-point moments, leave errors of order L/D."""
+    n = 24
+    s = np.linspace(0, 1, n + 1)
+    L = 2e-3
+    c0 = np.array([-0.2, 0.1, 0.05])
+    pts = c0 + L * np.stack([s, 0.3 * np.sin(3 * s), 0.2 * s * s], 1)
+    # the cell holds points 0..n-1; the segment of point n-1 ends at point n
+    t = _loop_tangle(np.vstack([pts, pts[::-1][1:-1] + np.array([0, 0, 0.5])]))
+    ids = np.arange(n)
+    centre = pts[:n].mean(axis=0)
+    direction = np.array([-0.2, 0.9, 0.39])
+    direction /= np.linalg.norm(direction)
+    dists = L * np.array([8, 16, 32, 64, 128])
+    e = _far_errors(oracle, t, ids, pts, False, centre, direction, dists, T.MP_DIPOLE)
+    ratios = e[1:] / e[:-1]
+    assert np.all((ratios > 0.18) & (ratios < 0.32)), (e, ratios)
