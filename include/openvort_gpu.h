@@ -256,6 +256,27 @@ int ov_gpu_rk4_run_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t0, do
 int ov_gpu_reconnection_candidates(ov_ctx *ctx, const ov_tangle_state *tangle, double rec_dist,
                                    double rec_angle, int *pairs, int max_pairs, int *n_pairs);
 
+/* Allocation-free variant of ov_gpu_velocities with OV_UPDATE_GEOMETRY
+ * (update_tangle, direct sum).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Every array is a
+ * caller-owned DEVICE array: pos, tangents, normals, vs, vels are three planes
+ * x[stride], y[stride], z[stride]; dxi one plane; fwd/rev the forward and
+ * reverse links of the n points as indices into those planes (every point a
+ * free point on a closed loop or a periodic line: no EMPTY or pinned entries).
+ * box and bimg are host structs (reference tangle.h:55-58, vec3_maths.h:61-68).
+ * Intermediate buffers (local induction, record stream, partial sums) live in
+ * one file-scope device array of 14.8 MiB per GPU, held until the process
+ * ends, so: stride must be at most 16384 points (larger is OV_E_INVALID),
+ * use_BH must be 0 (OV_E_UNSUPPORTED), and calls that overlap on one GPU race
+ * (they share that array).  Launches go on the context's stream; the call is
+ * synchronous on return and calls no allocating routine. */
+int ov_gpu_velocities_noalloc(ov_ctx *ctx, const double *pos, const int *fwd, const int *rev,
+                              int n, long stride, const ov_domain_box *box,
+                              const ov_boundary_images *bimg, double t, const ov_params *params,
+                              double *tangents, double *normals, double *dxi, double *vs,
+                              double *vels);
+
 /* Per-kernel timings gathered while OV_PROFILE was set; kernel names are the
  * device function names.  Returns the number of launches, 0 if none. */
 long ov_ctx_kernel_stats(ov_ctx *ctx, const char *kernel, double *total_ms);

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,9 @@ struct ov_ctx {
   std::vector<ImgDev> imgs;
   PhysDev ph{};
   int mixed = 0;
+  int n_img = 1;           // terms of the image sum (primary + boundary images)
+  int max_chunks = 256;    // partial-sum planes of the direct sum at most
+  unsigned char* scratch_ptr = nullptr;   // this device's `scratch` (no-alloc entry point)
   double rnear = 0.02;
   int wall_periodic_mask = 0;
   bool lia_only = false;
@@ -145,6 +149,47 @@ struct ov_ctx {
   unsigned long long *r_pairs = nullptr, *r_pairs_sorted = nullptr;
   long r_pair_cap = 0;
 };
+
+// ---- device scratch of the allocation-free entry point ------------------------
+// ov_gpu_velocities_noalloc keeps every buffer beyond its inputs and outputs
+// in one file-scope device array (each GPU the library runs on holds its own
+// copy until the process ends).  Layout for a plane stride of `cap` points:
+// LIA planes, the Biot-Savart sum, NOALLOC_CHUNKS partial-sum planes, the
+// record stream and its tiles, scan words, adjacency, images and a flag.
+namespace {
+constexpr long NOALLOC_MAX_POINTS = 16384;
+constexpr int NOALLOC_CHUNKS = 32;
+
+struct NoAllocLayout {
+  size_t lia, bs, partial, near, far, src, tgt, brk, rank, bsum, counts, adj, img, flag, total;
+};
+constexpr size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr NoAllocLayout noalloc_layout(long cap) {
+  NoAllocLayout L{};
+  const size_t max_rec = 2 * (size_t)cap + TILE;
+  size_t o = 0;
+  L.lia = o;     o = al256(o + sizeof(double) * 3 * (size_t)cap);
+  L.bs = o;      o = al256(o + sizeof(double) * 3 * (size_t)cap);
+  L.partial = o; o = al256(o + sizeof(double) * 3 * (size_t)cap * NOALLOC_CHUNKS);
+  L.near = o;    o = al256(o + sizeof(NearRec) * max_rec);
+  L.far = o;     o = al256(o + sizeof(FarRec) * max_rec);
+  L.src = o;     o = al256(o + sizeof(SrcTile) * (max_rec / TILE));
+  L.tgt = o;     o = al256(o + sizeof(TgtTile) * ((size_t)cap / TILE + 1));
+  L.brk = o;     o = al256(o + sizeof(int) * (size_t)cap);
+  L.rank = o;    o = al256(o + sizeof(int) * (size_t)cap);
+  L.bsum = o;    o = al256(o + sizeof(int) * ((size_t)cap / 256 + 1));
+  L.counts = o;  o = al256(o + sizeof(int) * 4);
+  L.adj = o;     o = al256(o + sizeof(int) * 2 * (size_t)cap);
+  L.img = o;     o = al256(o + sizeof(ImgDev) * MAX_IMAGES);
+  L.flag = o;    o = al256(o + sizeof(int));
+  L.total = o;
+  return L;
+}
+constexpr size_t NOALLOC_SCRATCH_BYTES = noalloc_layout(NOALLOC_MAX_POINTS).total;
+static_assert(NOALLOC_SCRATCH_BYTES < (size_t)20 << 20, "no-alloc scratch above 20 MiB");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -265,11 +310,82 @@ int ensure_host(ov_ctx* ctx, size_t bytes) {
   return OV_OK;
 }
 
+// Box, image list and physics of one call (no host or device allocation):
+// the image terms go to il[0..*nil) (primary first), the rest into ctx.
+int setup_box_physics(ov_ctx* ctx, const ov_domain_box* bx, const ov_boundary_images* bimg,
+                      const ov_params* p, ImgDev* il, int* nil) {
+  for (const ov_flow* f : {&p->vs, &p->vn, &p->vb})
+    if (f->type < OV_FLOW_NONE || f->type >= OV_FLOW_COUNT)
+      return fail(ctx, OV_E_INVALID, "unknown flow type %d", f->type);
+  if (bimg->n < 0 || bimg->n > MAX_IMAGES - 1 || (bimg->n > 0 && !bimg->images))
+    return fail(ctx, OV_E_INVALID, "bad image list (n=%d)", bimg->n);
+  for (int i = 0; i < 3; ++i) {
+    ctx->box.lo[i] = bx->bottom_left_back.p[i];
+    ctx->box.hi[i] = bx->top_right_front.p[i];
+    ctx->box.L[i] = ctx->box.hi[i] - ctx->box.lo[i];
+    ctx->box.periodic[i] = bx->wall[2 * i] == OV_WALL_PERIODIC;
+  }
+  ctx->wall_periodic_mask = 0;
+  for (int f = 0; f < 6; ++f)
+    if (bx->wall[f] == OV_WALL_PERIODIC) ctx->wall_periodic_mask |= 1 << f;
+  ImgDev prim{};
+  prim.axis = -1;
+  prim.primary = 1;
+  il[0] = prim;
+  int m = 1;
+  for (int j = 0; j < bimg->n; ++j) {
+    const ov_image_tangle& it = bimg->images[j];
+    if (it.reflect < -1 || it.reflect > 5)
+      return fail(ctx, OV_E_INVALID, "image %d: bad reflect %d", j, it.reflect);
+    ImgDev d{};
+    for (int i = 0; i < 3; ++i) d.shift[i] = ctx->box.L[i] * it.shift[i];
+    d.axis = -1;
+    if (it.reflect >= 0) {
+      int ax = it.reflect / 2;
+      double wall = (it.reflect & 1) ? ctx->box.hi[ax] : ctx->box.lo[ax];
+      d.axis = ax;
+      d.wall2 = 2 * wall;
+    }
+    d.primary = 0;
+    il[m++] = d;
+  }
+  *nil = m;
+  ctx->n_img = m;
+
+  PhysDev& ph = ctx->ph;
+  ph.kappa = p->kappa;
+  ph.a0 = p->vortex_width;
+  ph.alpha = p->alpha;
+  ph.alpha_p = p->alpha_p;
+  ph.dl_max = p->dl_max;
+  ph.max_curv = p->max_curvature_scale;
+  ph.hyperalpha = p->hyperalpha;
+  ph.vs = p->vs;
+  ph.vn = p->vn;
+  ctx->flow_time_dep = flow_time_dependent(p->vs.type) ||
+                       (p->use_mutual_friction && flow_time_dependent(p->vn.type));
+  ph.lia_only = p->LIA_only;
+  ph.use_mf = p->use_mutual_friction;
+  ph.hyperfriction = p->hyperfriction;
+  ctx->lia_only = p->LIA_only != 0;
+  ctx->mixed = p->precision == OV_PREC_MIXED;
+  ctx->use_tree = p->use_BH != 0;
+  ctx->op.box = ctx->box;
+  ctx->op.grain = p->BH_grain;
+  ctx->op.grain_dl = p->BH_grain * p->dl_max;
+  ctx->op.theta = p->BH_resolution;
+  ctx->op.quad = p->BH_quadtree != 0;
+  ctx->op.dipole = p->multipole == OV_MP_DIPOLE;
+  ctx->rnear = p->near_radius > 0 ? p->near_radius : 0.02;
+  return OV_OK;
+}
+
 // Validate the tangle the way the reference's walks require it and build the
 // stream order.  EMPTY points must have no forward link (the reference sums
 // every point whose forward link is set, tangle.c:612-615, and every EMPTY
 // point it creates has none, tangle.c:94-99, 1063-1067).
 int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
+  int rc = 0;
   if (!t || !p) return fail(ctx, OV_E_INVALID, "null tangle or params");
   if (t->N < 0) return fail(ctx, OV_E_INVALID, "negative N");
   if (t->N > 0 && (!t->vnodes || !t->connections || !t->status))
@@ -334,67 +450,15 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   ctx->act.swap(act_s);
   ctx->n = n;
 
-  // box and images
-  for (int i = 0; i < 3; ++i) {
-    ctx->box.lo[i] = t->box.bottom_left_back.p[i];
-    ctx->box.hi[i] = t->box.top_right_front.p[i];
-    ctx->box.L[i] = ctx->box.hi[i] - ctx->box.lo[i];
-    ctx->box.periodic[i] = t->box.wall[2 * i] == OV_WALL_PERIODIC;
-  }
-  ctx->wall_periodic_mask = 0;
-  for (int f = 0; f < 6; ++f)
-    if (t->box.wall[f] == OV_WALL_PERIODIC) ctx->wall_periodic_mask |= 1 << f;
-  ctx->imgs.clear();
-  ImgDev prim{};
-  prim.axis = -1;
-  prim.primary = 1;
-  ctx->imgs.push_back(prim);
-  for (int j = 0; j < t->bimg.n; ++j) {
-    const ov_image_tangle& it = t->bimg.images[j];
-    if (it.reflect < -1 || it.reflect > 5)
-      return fail(ctx, OV_E_INVALID, "image %d: bad reflect %d", j, it.reflect);
-    ImgDev d{};
-    for (int i = 0; i < 3; ++i) d.shift[i] = ctx->box.L[i] * it.shift[i];
-    d.axis = -1;
-    if (it.reflect >= 0) {
-      int ax = it.reflect / 2;
-      double wall = (it.reflect & 1) ? ctx->box.hi[ax] : ctx->box.lo[ax];
-      d.axis = ax;
-      d.wall2 = 2 * wall;
-    }
-    d.primary = 0;
-    ctx->imgs.push_back(d);
-  }
-
-  // physics
-  PhysDev& ph = ctx->ph;
-  ph.kappa = p->kappa;
-  ph.a0 = p->vortex_width;
-  ph.alpha = p->alpha;
-  ph.alpha_p = p->alpha_p;
-  ph.dl_max = p->dl_max;
-  ph.max_curv = p->max_curvature_scale;
-  ph.hyperalpha = p->hyperalpha;
-  ph.vs = p->vs;
-  ph.vn = p->vn;
-  ctx->flow_time_dep = flow_time_dependent(p->vs.type) ||
-                       (p->use_mutual_friction && flow_time_dependent(p->vn.type));
-  ph.lia_only = p->LIA_only;
-  ph.use_mf = p->use_mutual_friction;
-  ph.hyperfriction = p->hyperfriction;
-  ctx->lia_only = p->LIA_only != 0;
-  ctx->mixed = p->precision == OV_PREC_MIXED;
-  ctx->use_tree = p->use_BH != 0;
-  ctx->op.box = ctx->box;
-  ctx->op.grain = p->BH_grain;
-  ctx->op.grain_dl = p->BH_grain * p->dl_max;
-  ctx->op.theta = p->BH_resolution;
-  ctx->op.quad = p->BH_quadtree != 0;
-  ctx->op.dipole = p->multipole == OV_MP_DIPOLE;
+  // box, images and physics
+  ImgDev il[MAX_IMAGES];
+  int nil = 0;
+  if ((rc = setup_box_physics(ctx, &t->box, &t->bimg, p, il, &nil))) return rc;
+  ctx->imgs.assign(il, il + nil);
   ctx->rnear = p->near_radius > 0 ? p->near_radius : 0.02;
   ctx->profile = false;
 
-  int rc = ensure_capacity(ctx, std::max(n, 1));
+  rc = ensure_capacity(ctx, std::max(n, 1));
   if (rc) return rc;
   ctx->cap = ctx->buf_cap;
   if (ctx->imgs.size() > 0) {
@@ -765,7 +829,7 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
     const int n_tgt = (n + TILE - 1) / TILE;
     const int nb = (n + 255) / 256;
     const int max_src = (2 * n + TILE - 1) / TILE;
-    const int n_img = (int)ctx->imgs.size();
+    const int n_img = ctx->n_img;
     if ((rc = launch(ctx, "k_cont", k_cont, dim3(nb), dim3(256), X, (const int*)ctx->fwd, n, cap,
                      ctx->box, ctx->brk, ctx->rank, ctx->bsum)))
       return rc;
@@ -788,7 +852,7 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
     // but no more chunks than work items per target tile (each chunk's
     // partial plane costs 24 bytes per point to write and to reduce)
     long chunks = (65536 + n_tgt - 1) / n_tgt;
-    chunks = std::max<long>(1, std::min<long>(chunks, std::min<long>(256, (long)n_tgt * n_img)));
+    chunks = std::max<long>(1, std::min<long>(chunks, std::min<long>(ctx->max_chunks, (long)n_tgt * n_img)));
     size_t pbytes = sizeof(double) * 3 * (size_t)cap * (size_t)chunks;
     if (pbytes > ctx->partial_bytes) {
       if ((rc = dalloc(ctx, (void**)&ctx->partial, pbytes))) return rc;
@@ -875,6 +939,11 @@ int rk4_device(ov_ctx* ctx, double t, double dt, int rk_variant, bool caller_k1,
   return launch(ctx, "k_combine", k_combine, grid1(n), dim3(256), ctx->X0, (const double*)ctx->K[0],
                 (const double*)ctx->K[1], (const double*)ctx->K[2], (const double*)ctx->K[3], dt, n,
                 cap, ctx->flag);
+}
+
+int mark_begin_stream(ov_ctx* ctx, hipEvent_t ev) {
+  HIPCHK(hipEventRecord(ev, ctx->stream));
+  return OV_OK;
 }
 
 int mark_begin(ov_ctx* ctx) {
@@ -1227,6 +1296,23 @@ int rec_pair_buffers(ov_ctx* ctx, long need) {
   return OV_OK;
 }
 
+// Device address of `scratch` per device, looked up once per device with
+// that device current (hipGetSymbolAddress); the array's name is only a
+// host-side stand-in and is never passed to a kernel or a copy.
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {nullptr};
+
+void* scratch_address(int device) {
+  if (device < 0 || device >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1268,6 +1354,8 @@ ov_ctx* ov_ctx_create(int device) {
     delete ctx;
     return nullptr;
   }
+  // set-up of the allocation-free entry point: this device's scratch address
+  ctx->scratch_ptr = (unsigned char*)scratch_address(device);
   return ctx;
 }
 
@@ -1937,6 +2025,85 @@ int ov_gpu_reconnection_candidates(ov_ctx* ctx, const ov_tangle_state* t, double
       pairs[2 * i + 1] = (int)(h[i] & 0xffffffffu);
     }
   }
+  return OV_OK;
+}
+
+// update_tangle + update_velocities (the direct sum) on device-resident
+// planes, with every intermediate buffer in the file-scope scratch array: no
+// device or host allocation.  See include/openvort_gpu.h.
+int ov_gpu_velocities_noalloc(ov_ctx* ctx, const double* pos, const int* fwd, const int* rev, int n,
+                              long stride, const ov_domain_box* box, const ov_boundary_images* bimg,
+                              double time, const ov_params* p, double* tangents, double* normals,
+                              double* dxi, double* vs, double* vels) {
+  if (!ctx) return OV_E_INVALID;
+  if (!box || !bimg || !p) return fail(ctx, OV_E_INVALID, "noalloc: null box, images or params");
+  if (n < 0 || stride < n) return fail(ctx, OV_E_INVALID, "noalloc: n = %d, stride = %ld", n, stride);
+  if (stride > NOALLOC_MAX_POINTS)
+    return fail(ctx, OV_E_INVALID, "noalloc: stride %ld above the maximum of %ld points", stride,
+                NOALLOC_MAX_POINTS);
+  if (p->use_BH) return fail(ctx, OV_E_UNSUPPORTED, "noalloc: direct sum only (use_BH = 0)");
+  if (n == 0) return OV_OK;
+  if (!pos || !fwd || !rev || !tangents || !normals || !dxi || !vs || !vels)
+    return fail(ctx, OV_E_INVALID, "noalloc: null device array");
+  if (!ctx->scratch_ptr) return fail(ctx, OV_E_DEVICE, "noalloc: no scratch address on device %d", ctx->device);
+  HIPCHK(hipSetDevice(ctx->device));
+
+  // a view of the context whose buffers all lie in this device's scratch
+  ov_ctx v;
+  v.device = ctx->device;
+  v.stream = ctx->stream;
+  ImgDev il[MAX_IMAGES];
+  int nil = 0;
+  int rc = setup_box_physics(&v, box, bimg, p, il, &nil);
+  if (rc) {
+    memcpy(ctx->err, v.err, sizeof ctx->err);
+    return rc;
+  }
+  if (v.use_tree) return fail(ctx, OV_E_UNSUPPORTED, "noalloc: direct sum only (use_BH = 0)");
+  v.n = n;
+  v.cap = stride;
+  v.max_chunks = NOALLOC_CHUNKS;
+  v.profile = false;
+  const NoAllocLayout L = noalloc_layout(NOALLOC_MAX_POINTS);
+  unsigned char* b = ctx->scratch_ptr;
+  v.fwd = const_cast<int*>(fwd);
+  v.rev = const_cast<int*>(rev);
+  v.lia = (double*)(b + L.lia);
+  v.bs = (double*)(b + L.bs);
+  v.partial = (double*)(b + L.partial);
+  v.partial_bytes = L.near - L.partial;
+  v.near = (NearRec*)(b + L.near);
+  v.far = (FarRec*)(b + L.far);
+  v.src = (SrcTile*)(b + L.src);
+  v.tgt = (TgtTile*)(b + L.tgt);
+  v.brk = (int*)(b + L.brk);
+  v.rank = (int*)(b + L.rank);
+  v.bsum = (int*)(b + L.bsum);
+  v.counts = (int*)(b + L.counts);
+  v.adj = (int*)(b + L.adj);
+  v.img = (ImgDev*)(b + L.img);
+  v.flag = (int*)(b + L.flag);
+  ImgList lst;
+  for (int i = 0; i < nil; ++i) lst.im[i] = il[i];
+  if (!rc) rc = launch(&v, "k_set_images", k_set_images, dim3(1), dim3(64), lst, nil, v.img);
+  if (!rc) rc = launch(&v, "k_zero_int", k_zero_int, dim3(1), dim3(64), v.flag);
+  if (!rc) rc = mark_begin_stream(&v, ctx->t_begin);
+  if (!rc) rc = evaluate(&v, pos, true, tangents, normals, dxi, vs, vels, time);
+  int flag = 0;
+  if (!rc) {
+    hipError_t e = hipEventRecord(ctx->t_end, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, v.flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = fail(&v, OV_E_DEVICE, "noalloc: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    memcpy(ctx->err, v.err, sizeof ctx->err);
+    return rc;
+  }
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, ctx->t_begin, ctx->t_end) == hipSuccess) ctx->last_device_ms = ms;
+  ctx->last_interactions = v.last_interactions;
+  if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite velocity");
   return OV_OK;
 }
 

@@ -1086,6 +1086,21 @@ __device__ __forceinline__ void wrap_point(int s, double* __restrict__ X0, long 
   store3(X0, cap, s, x);
 }
 
+// Image list passed by value (kernel argument), written to device memory by
+// k_set_images: a host-to-device copy without a staging buffer.
+struct ImgList {
+  ImgDev im[MAX_IMAGES];
+};
+__global__ void __launch_bounds__(64) k_set_images(ImgList L, int n, ImgDev* __restrict__ dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = L.im[i];
+}
+
+// Zero one int (a fill kernel, not a memset node, so it may be captured).
+__global__ void __launch_bounds__(64) k_zero_int(int* __restrict__ p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
 __global__ void __launch_bounds__(256) k_wrap(double* __restrict__ X0, int n, long cap, BoxDev box,
                                               int wall_periodic_mask) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;

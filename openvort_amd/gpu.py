@@ -39,6 +39,7 @@ SYMBOLS = (
     "ov_gpu_velocity_vjp",
     "ov_gpu_rk4_run_vjp",
     "ov_gpu_reconnection_candidates",
+    "ov_gpu_velocities_noalloc",
 )
 
 _lib = None
@@ -102,6 +103,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.ov_gpu_reconnection_candidates.argtypes = [C.c_void_p, P(TangleState), C.c_double, C.c_double,
                                                    C.c_void_p, C.c_int, P(C.c_int)]
     lib.ov_gpu_reconnection_candidates.restype = C.c_int
+    lib.ov_gpu_velocities_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                              C.c_long, C.c_void_p, C.c_void_p, C.c_double, P(Params),
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]
+    lib.ov_gpu_velocities_noalloc.restype = C.c_int
     _lib = lib
     return lib
 
@@ -215,6 +221,17 @@ class Context:
             if n.value <= cap:
                 return buf[:n.value].copy()
             cap = n.value
+
+    def velocities_noalloc(self, pos, fwd, rev, n: int, stride: int, t: Tangle, time: float,
+                           params: Params, tangents, normals, dxi, vs, vels) -> int:
+        """ov_gpu_velocities_noalloc on device arrays given as integer
+        addresses (planes of `stride` doubles; int32 links); t supplies the
+        box and the image list only."""
+        ts = t.cstruct()
+        rc = self.lib.ov_gpu_velocities_noalloc(self.ptr, pos, fwd, rev, n, stride,
+                                                C.addressof(ts.box), C.addressof(ts.bimg), time,
+                                                C.byref(params), tangents, normals, dxi, vs, vels)
+        return self._check(rc)
 
     def batch_tuning(self):
         u, b, ms = C.c_int(), C.c_int(), C.c_double()
