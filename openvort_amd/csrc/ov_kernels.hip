@@ -705,11 +705,16 @@ constexpr double RSQ_BIAS_FIX = 1.0 + 2.654e-8;
 
 // One FP32 small-angle pair (the FP32 arithmetic of far_tile_fast, written
 // with explicit fusions so that a recomputation gives the same bits):
-// c = R_prev x d/2, s4 = s/4, f = 2f.
+// c = R_prev x d/2 and g = s/4 - ZT, with the zero rule's threshold ZT =
+// 5e-9 (s < 2e-8) folded into the fused multiply-add that forms s/4, so that
+// the rule is g < 0 and its band |g| < 2.5e-12.  f = b (1 + g) is 2f scaled by
+// 1 - ZT / (1 + s/4) (to within 5e-13 of the constant 1 - ZT, which the
+// tile's sum takes back).
 struct PairS {
   float R[3], l2, u;
-  float c[3], f, s4;
+  float c[3], f, g;
 };
+constexpr float SMALL_ZT = 5e-9f;
 __device__ __forceinline__ PairS pair_small(const float Rp[3], float up, const FarRec& q,
                                             const float off[3]) {
 #pragma clang fp contract(off)
@@ -724,14 +729,17 @@ __device__ __forceinline__ PairS pair_small(const float Rp[3], float up, const F
   o.c[2] = fmaf(Rp[0], q.d[1], -(Rp[1] * q.d[0]));
   const float c2 = fmaf(o.c[2], o.c[2], fmaf(o.c[1], o.c[1], o.c[0] * o.c[0]));
   const float w = up * o.u;
-  o.s4 = c2 * (w * w);
+  o.g = fmaf(c2, w * w, -SMALL_ZT);
   const float b = (up + o.u) * w;
-  o.f = fmaf(b, o.s4, b);
+  o.f = fmaf(b, o.g, b);
   return o;
 }
 // zero rule s < 2e-8 and the band re-decided exactly, in units of s/4
-__device__ __forceinline__ bool small_zero(float s4) { return s4 < 5e-9f; }
-__device__ __forceinline__ bool small_band(float s4) { return fabsf(s4 - 5e-9f) < 2.5e-12f; }
+__device__ __forceinline__ bool small_zero(float g) { return g < 0.f; }
+__device__ __forceinline__ bool small_band(float g) { return fabsf(g) < 2.5e-12f; }
+// per-tile scale of the fast pass: the mean v_rsq_f32 bias and the folded
+// threshold (1 + ZT)
+constexpr double SMALL_TILE_SCALE = RSQ_BIAS_FIX * (1.0 + (double)SMALL_ZT);
 
 // FP32 pass over one source tile at small angles (far_tile without the
 // reciprocal).  Zero-rule band detection: each record's band test becomes a
@@ -739,7 +747,7 @@ __device__ __forceinline__ bool small_band(float s4) { return fabsf(s4 - 5e-9f) 
 // group is in a band are that group's records re-evaluated (pair_small gives
 // the same bits) to note which lanes were in a band and what they decided.
 // acc receives this tile's sum only (it starts at zero and is scaled by
-// RSQ_BIAS_FIX at the end).
+// SMALL_TILE_SCALE at the end).
 __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
                                               const NearRec* __restrict__ nrec, const SrcTile& st,
                                               int base, const float off[3], const double rs[3],
@@ -759,11 +767,11 @@ __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const PairS o = pair_small(Rp, up, q[k], off);
-      const float f = small_zero(o.s4) ? 0.f : o.f;
+      const float f = small_zero(o.g) ? 0.f : o.f;
       a8[0] = fmaf(f, o.c[0], a8[0]);
       a8[1] = fmaf(f, o.c[1], a8[1]);
       a8[2] = fmaf(f, o.c[2], a8[2]);
-      any |= __builtin_amdgcn_ballot_w64(small_band(o.s4));
+      any |= __builtin_amdgcn_ballot_w64(small_band(o.g));
       Rp[0] = o.R[0];
       Rp[1] = o.R[1];
       Rp[2] = o.R[2];
@@ -778,10 +786,10 @@ __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const PairS o = pair_small(Rb, ub, q[k], off);
-        const bool band = small_band(o.s4);
+        const bool band = small_band(o.g);
         if (__builtin_amdgcn_ballot_w64(band) != 0) bandmask |= 1ull << (e0 + k);
         if (band) lane_band |= 1ull << (e0 + k);
-        if (band && small_zero(o.s4)) lane_zero |= 1ull << (e0 + k);
+        if (band && small_zero(o.g)) lane_zero |= 1ull << (e0 + k);
         Rb[0] = o.R[0];
         Rb[1] = o.R[1];
         Rb[2] = o.R[2];
@@ -817,9 +825,9 @@ __device__ __forceinline__ void far_tile_fast(const FarRec* __restrict__ rec,
       }
     }
   }
-  acc[0] *= RSQ_BIAS_FIX;
-  acc[1] *= RSQ_BIAS_FIX;
-  acc[2] *= RSQ_BIAS_FIX;
+  acc[0] *= SMALL_TILE_SCALE;
+  acc[1] *= SMALL_TILE_SCALE;
+  acc[2] *= SMALL_TILE_SCALE;
 }
 
 // A lane's target: position, stream index, whether it is a real point, and
