@@ -840,7 +840,8 @@ struct LaneTarget {
 };
 
 // Work items [w0, w1) of one target tile: item w = (source tile J_base +
-// w / n_img, image w % n_img).  MIXED: FP32 pass for source tiles beyond the
+// w / n_img, image w % n_img), walked with an incremental (tile, image)
+// counter; the mirror axis is wave-uniform, so it selects by scalar branches.  MIXED: FP32 pass for source tiles beyond the
 // near radius; otherwise all FP64.  Adds the raw sums (without kappa/4pi).
 template <bool MIXED>
 __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
@@ -849,9 +850,13 @@ __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
                                              const ImgDev* __restrict__ img, int n_img, int J_base,
                                              const TgtTile& tt, const LaneTarget& L, long w0,
                                              long w1, double rnear2, double acc[3]) {
-  for (long w = w0; w < w1; ++w) {
-    const int Jl = (int)(w / n_img);
-    const int I = (int)(w - (long)Jl * n_img);
+  int Jl = (int)(w0 / n_img);
+  int I = (int)(w0 - (long)Jl * n_img);
+  for (long w = w0; w < w1; ++w, ++I) {
+    if (I == n_img) {
+      I = 0;
+      ++Jl;
+    }
     const int J = J_base + Jl;
     const ImgDev im = img[I];
     const SrcTile sj = src[J];
@@ -863,12 +868,15 @@ __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
       tlo[i] = tt.lo[i] - im.shift[i];
       thi[i] = tt.hi[i] - im.shift[i];
     }
-    if (im.axis >= 0) {
-      const int ax = im.axis;
-      rs[ax] = im.wall2 - L.r[ax];
-      double l = im.wall2 - tt.hi[ax], h = im.wall2 - tt.lo[ax];
-      tlo[ax] = l;
-      thi[ax] = h;
+    if (__builtin_expect(im.axis >= 0, 0)) {
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+        if (ax == im.axis) {
+          rs[ax] = im.wall2 - L.r[ax];
+          double l = im.wall2 - tt.hi[ax], h = im.wall2 - tt.lo[ax];
+          tlo[ax] = l;
+          thi[ax] = h;
+        }
     }
     double d2 = 0;
 #pragma unroll
@@ -892,7 +900,11 @@ __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
     } else {
       near_tile(near, sj, J * TILE, rs, L.s, im.primary != 0, v);
     }
-    if (im.axis >= 0) v[im.axis] = -v[im.axis];
+    if (__builtin_expect(im.axis >= 0, 0)) {
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+        if (ax == im.axis) v[ax] = -v[ax];
+    }
     acc[0] += v[0];
     acc[1] += v[1];
     acc[2] += v[2];

@@ -42,6 +42,13 @@ class Topo:
         # collinearity cut-off of the segment law (src/tangle.c:505-506); the
         # reference's 1e-8 unless a check sets it otherwise
         self.ndot_tol = 1e-8
+        # when a list: the number of pairs the cut-off zeroes, per image sum
+        self.cut_counts = None
+
+
+def zero_adj(adjacent, primary):
+    """the pairs the primary sum skips anyway"""
+    return adjacent if primary else torch.zeros_like(adjacent)
 
 
 def pwrap(r1, r2, topo):
@@ -106,9 +113,12 @@ def biot_savart(X, topo):
             l = torch.linalg.norm(R, dim=2)
             lp = torch.linalg.norm(Rp, dim=2)
             nd = (R * Rp).sum(2) / l / lp
-            zero = (l < 1e-8) | (lp < 1e-8) | ((nd - 1).abs() < topo.ndot_tol)
+            cut = (nd - 1).abs() < topo.ndot_tol
+            zero = (l < 1e-8) | (lp < 1e-8) | cut
             if primary:
                 zero = zero | adjacent
+            if topo.cut_counts is not None:
+                topo.cut_counts.append(int((cut & ~zero_adj(adjacent, primary)).sum()))
         keep = ~zero
         Rs = torch.where(keep[..., None], R, one)
         Rps = torch.where(keep[..., None], Rp, two)
