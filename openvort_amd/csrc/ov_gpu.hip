@@ -24,6 +24,7 @@
 #include "ov_batch.hip"
 #include "ov_adjoint.hip"
 #include "ov_reconnect.hip"
+#include "ov_host.hip"
 
 #include <hipcub/hipcub.hpp>
 

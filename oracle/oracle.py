@@ -178,6 +178,14 @@ class Reference:
             lib.ovref_reconnect_candidates.argtypes = [P(TangleState), C.c_double, P(C.c_int), C.c_long]
             lib.ovref_reconnect_candidates.restype = C.c_int
             lib.ovref_flow.argtypes = [P(Flow), P(C.c_double), C.c_double, P(C.c_double)]
+            lib.ovref_save_tangle.argtypes = [P(TangleState), C.c_char_p]
+            lib.ovref_save_tangle.restype = None
+            lib.ovref_load_tangle.argtypes = [C.c_char_p, C.c_int, P(C.c_double), P(C.c_int)]
+            lib.ovref_load_tangle.restype = C.c_int
+            lib.ovref_vconf_size.argtypes = []
+            lib.ovref_vconf_size.restype = C.c_int
+            lib.ovref_vconf.argtypes = [P(Flow), C.c_void_p]
+            lib.ovref_vconf.restype = C.c_int
             Reference._lib = lib
         self.lib = Reference._lib
         self.threads = threads
@@ -242,3 +250,25 @@ class Reference:
         """The reference's own flow function (its table entry for f's type,
         parameters set by name) at position r, time t."""
         return _flow_call(self.lib.ovref_flow, f, r, t)
+
+    def save_tangle(self, t: Tangle, path: str):
+        """The reference's save_tangle."""
+        ts = t.cstruct()
+        self.lib.ovref_save_tangle(C.byref(ts), path.encode())
+
+    def load_tangle(self, path: str, n: int):
+        """The reference's load_tangle into a tangle of capacity n: (rc,
+        positions (n, 3), links (n, 2) forward/reverse)."""
+        pos = np.zeros((n, 3))
+        links = np.zeros((n, 2), dtype=np.int32)
+        rc = self.lib.ovref_load_tangle(path.encode(), n, pos.ctypes.data_as(P(C.c_double)),
+                                        links.ctypes.data_as(P(C.c_int)))
+        return rc, pos, links
+
+    def vconf(self, f: Flow):
+        """Bytes of the reference's v_conf_t for flow f (the buffer must stay
+        alive while its address is used)."""
+        buf = C.create_string_buffer(self.lib.ovref_vconf_size())
+        if self.lib.ovref_vconf(C.byref(f), buf) != 0:
+            raise RuntimeError("unknown flow")
+        return buf

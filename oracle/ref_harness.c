@@ -19,6 +19,7 @@
 #include "vortex_dynamics.h"
 
 #include "../include/openvort_gpu.h"
+#include "../include/openvort_host.h"
 
 _Static_assert(sizeof(ov_tangle_state) == sizeof(struct tangle_state), "layout");
 _Static_assert(offsetof(ov_tangle_state, connections) == offsetof(struct tangle_state, connections), "layout");
@@ -27,6 +28,9 @@ _Static_assert(offsetof(ov_tangle_state, bimg) == offsetof(struct tangle_state, 
 _Static_assert(offsetof(ov_tangle_state, N) == offsetof(struct tangle_state, N), "layout");
 _Static_assert(sizeof(ov_node_status) == sizeof(node_status), "layout");
 _Static_assert(sizeof(ov_image_tangle) == sizeof(struct image_tangle), "layout");
+_Static_assert(sizeof(ov_v_conf) == sizeof(struct v_conf_t), "layout");
+_Static_assert(offsetof(ov_v_conf, v_params) == offsetof(struct v_conf_t, v_params), "layout");
+_Static_assert(sizeof(ov_v_param) == sizeof(struct v_param_t), "layout");
 
 extern int global_num_threads;
 
@@ -202,4 +206,41 @@ int ovref_reconnect_candidates(ov_tangle_state *t, double time, const int *pairs
     i = next_k;
   }
   return recs;
+}
+
+/* ---- frames and flow configurations (host I/O checks) --------------------- */
+
+#include "vortex_utils.h"
+
+/* The reference's save_tangle (vortex_utils.c:306-352) on t. */
+void ovref_save_tangle(ov_tangle_state *t, const char *path) {
+  save_tangle(path, (struct tangle_state *)t);
+}
+
+/* The reference's load_tangle (vortex_utils.c:354-387) of path into a tangle
+ * of capacity n it creates itself; the first n positions and links are
+ * copied out ([n][3] doubles, [n][2] ints).  Returns load_tangle's value. */
+int ovref_load_tangle(const char *path, int n, double *pos, int *links) {
+  struct tangle_state t;
+  create_tangle(&t, (size_t)n);
+  int rc = load_tangle(path, &t);
+  for (int k = 0; k < n && k < t.N; ++k) {
+    for (int i = 0; i < 3; ++i) pos[3 * k + i] = t.vnodes[k].p[i];
+    links[2 * k] = t.connections[k].forward;
+    links[2 * k + 1] = t.connections[k].reverse;
+  }
+  free_tangle(&t);
+  return rc;
+}
+
+/* sizeof(struct v_conf_t) */
+int ovref_vconf_size(void) { return (int)sizeof(struct v_conf_t); }
+
+/* The reference's v_conf_t for f (the table entry, parameters by name) into
+ * out, which holds ovref_vconf_size() bytes. */
+int ovref_vconf(const ov_flow *f, void *out) {
+  struct v_conf_t c;
+  if (set_flow(&c, f)) return -1;
+  memcpy(out, &c, sizeof c);
+  return 0;
 }
