@@ -256,6 +256,67 @@ int ov_gpu_rk4_run_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t0, do
 int ov_gpu_reconnection_candidates(ov_ctx *ctx, const ov_tangle_state *tangle, double rec_dist,
                                    double rec_angle, int *pairs, int max_pairs, int *n_pairs);
 
+/* ---- topology edits (reference src/tangle.c:766-1070) -------------------- */
+
+/* Settings of remesh and eliminate_small_loops: the reference globals
+ * global_dl_min, global_dl_max, global_small_loop_cutoff and
+ * global_eliminate_* (vortex_constants.c:39-66). */
+typedef struct {
+  double dl_min, dl_max;                  /* remesh(min_dist, max_dist) */
+  int small_loop_cutoff;                  /* eliminate_small_loops(loop_length) */
+  int eliminate_origin_loops;             /* whole loops within the cutoff of the origin */
+  double eliminate_loops_origin_cutoff;
+  int eliminate_zaxis_loops;              /* whole loops within the cutoff of the z axis */
+  double eliminate_loops_zaxis_cutoff;
+  int eliminate_outer_loops;              /* whole loops beyond the cutoff from the z axis */
+  double eliminate_outer_loops_cutoff;
+} ov_topology;
+
+/* The reference defaults (dl_min 1e-3, dl_max 5e-3, small_loop_cutoff 5,
+ * eliminations off with cutoffs 3e-2, 2e-2, 1). */
+void ov_topology_default(ov_topology *tp);
+
+/* remesh(tangle, min_dist, max_dist) (tangle.c:824-873) on the GPU: points
+ * visited in index order, a point removed (remove_point, tangle.c:1017-1066)
+ * when a neighbouring segment is shorter than min_dist and the two together
+ * shorter than max_dist, one added (add_point, tangle.c:1069-1160, at the
+ * slot get_tangle_next_free gives) after a segment longer than max_dist, the
+ * geometry refreshed after every change and the periodic wrap applied when
+ * points were added.  Reads the caller's dxi / tangents / normals (current,
+ * as the reference requires); writes vnodes, links, status, dxi, tangents,
+ * normals and next_free.  Where the reference would expand_tangle (no EMPTY
+ * slot left) it returns OV_E_NOMEM and leaves the tangle unchanged: pass a
+ * tangle with spare capacity.  add_point's curvature test uses
+ * params->max_curvature_scale / params->dl_max (global_max_curvature_scale,
+ * global_dl_max). */
+int ov_gpu_remesh(ov_ctx *ctx, ov_tangle_state *tangle, const ov_params *params,
+                  double min_dist, double max_dist);
+
+/* eliminate_small_loops(tangle, tp->small_loop_cutoff) (tangle.c:878-945)
+ * with the origin / z-axis / outer eliminations it runs first when enabled
+ * (tangle.c:947-1015): every loop of fewer than small_loop_cutoff + 1
+ * points, or lying wholly inside (outside) the cutoff, is removed.  Writes
+ * links and status.  The reference's use of `recalculate` as scratch marks
+ * is not reproduced. */
+int ov_gpu_eliminate_small_loops(ov_ctx *ctx, ov_tangle_state *tangle, const ov_topology *tp);
+
+/* enforce_boundaries (tangle.c:796-819): periodic wrap of every point. */
+int ov_gpu_enforce_boundaries(ov_ctx *ctx, ov_tangle_state *tangle);
+
+/* nsteps of the reference's main loop body (main.c:112-117) with the tangle
+ * resident on the device: update_tangle(t); rk4_step(t, dt);
+ * update_tangents_normals; remesh(dl_min, dl_max); eliminate_small_loops;
+ * enforce_boundaries; t += dt.  The point stream is rebuilt on the device
+ * every step (loops in order of their lowest index, as every other entry
+ * point orders them), so each step equals ov_gpu_rk4_step followed by the
+ * three edits above.  Writes vnodes, links, status, next_free, the geometry
+ * after the last step's edits, and vels / vs of the last step's k1 (slots
+ * created afterwards keep the values the slot held, as in the reference).
+ * OV_E_NOMEM as ov_gpu_remesh when the capacity runs out (nothing written).
+ * Direct sum only.  Flags: OV_PROFILE. */
+int ov_gpu_run(ov_ctx *ctx, ov_tangle_state *tangle, double t0, double dt, int nsteps,
+               const ov_params *params, const ov_topology *tp, int flags);
+
 /* Allocation-free variant of ov_gpu_velocities with OV_UPDATE_GEOMETRY
  * (update_tangle, direct sum).  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,

@@ -24,6 +24,7 @@
 #include "ov_batch.hip"
 #include "ov_adjoint.hip"
 #include "ov_reconnect.hip"
+#include "ov_topology.hip"
 #include "ov_host.hip"
 
 #include <hipcub/hipcub.hpp>
@@ -149,6 +150,17 @@ struct ov_ctx {
   long r_cap = 0;
   unsigned long long *r_pairs = nullptr, *r_pairs_sorted = nullptr;
   long r_pair_cap = 0;
+
+  // topology edits: the tangle in the caller's index space (capacity ix_N)
+  int ix_N = 0;
+  long ix_alloc = 0;
+  double *ix_X = nullptr, *ix_T = nullptr, *ix_N3 = nullptr, *ix_dxi = nullptr, *ix_vels = nullptr,
+         *ix_vs = nullptr;
+  int *ix_fwd = nullptr, *ix_rev = nullptr, *ix_status = nullptr;
+  int *ix_lab[2] = {nullptr, nullptr}, *ix_jmp[2] = {nullptr, nullptr}, *ix_d[2] = {nullptr, nullptr};
+  int *ix_len = nullptr, *ix_off = nullptr, *ix_pos = nullptr, *ix_act = nullptr, *ix_bits = nullptr,
+      *ix_all = nullptr, *ix_state = nullptr;
+  std::vector<int> ix_status_in;   // caller status on entry (new points get pin_wall NOT_A_FACE)
 };
 
 // ---- device scratch of the allocation-free entry point ------------------------
@@ -1314,6 +1326,291 @@ void* scratch_address(int device) {
   return g_scratch_ptr[device];
 }
 
+// ---- topology edits in the caller's index space --------------------------------
+
+int ix_buffers(ov_ctx* ctx, int N) {
+  if (N <= ctx->ix_alloc) return OV_OK;
+  const long c = std::max<long>(N, 64);
+  const size_t v3 = sizeof(double) * 3 * (size_t)c, v1 = sizeof(double) * (size_t)c, i1 = sizeof(int) * (size_t)c;
+  int rc;
+  double** d3[] = {&ctx->ix_X, &ctx->ix_T, &ctx->ix_N3, &ctx->ix_vels, &ctx->ix_vs};
+  for (double** b : d3)
+    if ((rc = dalloc(ctx, (void**)b, v3))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->ix_dxi, v1))) return rc;
+  int** i1b[] = {&ctx->ix_fwd, &ctx->ix_rev, &ctx->ix_status, &ctx->ix_lab[0], &ctx->ix_lab[1],
+                 &ctx->ix_jmp[0], &ctx->ix_jmp[1], &ctx->ix_d[0], &ctx->ix_d[1], &ctx->ix_len,
+                 &ctx->ix_off, &ctx->ix_pos, &ctx->ix_act, &ctx->ix_bits, &ctx->ix_all};
+  for (int** b : i1b)
+    if ((rc = dalloc(ctx, (void**)b, i1))) return rc;
+  if ((rc = dalloc(ctx, (void**)&ctx->ix_state, sizeof(int) * 8))) return rc;
+  ctx->ix_alloc = c;
+  return OV_OK;
+}
+
+// Validate a tangle for the index-space edits (free points on closed loops,
+// consistent links; pinned points are not supported) and set box/physics.
+int ix_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
+  if (!t) return fail(ctx, OV_E_INVALID, "null tangle");
+  const int N = t->N;
+  if (N < 0) return fail(ctx, OV_E_INVALID, "negative N");
+  if (N > 0 && (!t->vnodes || !t->connections || !t->status))
+    return fail(ctx, OV_E_INVALID, "null tangle arrays");
+  for (int k = 0; k < N; ++k) {
+    const int st = t->status[k].status;
+    if (st == OV_EMPTY) continue;
+    if (st == OV_PINNED || st == OV_PINNED_SLIP)
+      return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are not supported", k);
+    if (st != OV_FREE) return fail(ctx, OV_E_INVALID, "bad status %d at %d", st, k);
+    const int f = t->connections[k].forward, r = t->connections[k].reverse;
+    if (f < 0 || f >= N || r < 0 || r >= N || t->status[f].status != OV_FREE ||
+        t->status[r].status != OV_FREE || t->connections[f].reverse != k)
+      return fail(ctx, OV_E_INVALID, "point %d: links %d/%d inconsistent", k, f, r);
+  }
+  ov_params q;
+  if (p)
+    q = *p;
+  else
+    ov_params_default(&q);
+  ImgDev il[MAX_IMAGES];
+  int nil = 0;
+  int rc = setup_box_physics(ctx, &t->box, &t->bimg, &q, il, &nil);
+  if (rc) return rc;
+  ctx->imgs.assign(il, il + nil);
+  ctx->ix_N = N;
+  return ix_buffers(ctx, N);
+}
+
+// caller AoS -> index-space planes (vec3) / scalars, through the staging buffer
+int ix_up3(ov_ctx* ctx, const ov_vec3d* src, double* dst) {
+  const int N = ctx->ix_N;
+  const long c = ctx->ix_alloc;
+  int rc = ensure_host(ctx, sizeof(double) * 3 * (size_t)c);
+  if (rc) return rc;
+  double* h = ctx->hst;
+  for (int k = 0; k < N; ++k)
+    for (int i = 0; i < 3; ++i) h[(size_t)i * c + k] = src ? src[k].p[i] : 0.0;
+  HIPCHK(hipMemcpyAsync(dst, h, sizeof(double) * 3 * (size_t)c, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return OV_OK;
+}
+int ix_down3(ov_ctx* ctx, const double* src, ov_vec3d* dst, const int* only) {
+  const int N = ctx->ix_N;
+  const long c = ctx->ix_alloc;
+  int rc = ensure_host(ctx, sizeof(double) * 3 * (size_t)c);
+  if (rc) return rc;
+  double* h = ctx->hst;
+  HIPCHK(hipMemcpyAsync(h, src, sizeof(double) * 3 * (size_t)c, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < N; ++k)
+    if (!only || only[k] != OV_EMPTY)
+      for (int i = 0; i < 3; ++i) dst[k].p[i] = h[(size_t)i * c + k];
+  return OV_OK;
+}
+
+// Upload positions, links, status and (with geom) dxi / s' / s'' and
+// vels / vs of the caller's tangle into the index-space arrays.
+int ix_upload(ov_ctx* ctx, const ov_tangle_state* t, bool geom, bool vel) {
+  const int N = ctx->ix_N;
+  int rc;
+  if ((rc = ix_up3(ctx, t->vnodes, ctx->ix_X))) return rc;
+  if (geom) {
+    if (!t->tangents || !t->normals || !t->dxi) return fail(ctx, OV_E_INVALID, "geometry arrays required");
+    if ((rc = ix_up3(ctx, t->tangents, ctx->ix_T)) || (rc = ix_up3(ctx, t->normals, ctx->ix_N3))) return rc;
+  }
+  if (vel) {
+    if ((rc = ix_up3(ctx, t->vels, ctx->ix_vels)) || (rc = ix_up3(ctx, t->vs, ctx->ix_vs))) return rc;
+  }
+  std::vector<int> f(N > 0 ? N : 1), r(N > 0 ? N : 1), st(N > 0 ? N : 1);
+  std::vector<double> dx(N > 0 ? N : 1, 0.0);
+  ctx->ix_status_in.assign(N, OV_EMPTY);
+  for (int k = 0; k < N; ++k) {
+    f[k] = t->connections[k].forward;
+    r[k] = t->connections[k].reverse;
+    st[k] = t->status[k].status == OV_EMPTY ? OV_EMPTY : OV_FREE;
+    ctx->ix_status_in[k] = st[k];
+    if (geom) dx[k] = t->dxi[k];
+  }
+  if (N > 0) {
+    HIPCHK(hipMemcpyAsync(ctx->ix_fwd, f.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->ix_rev, r.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->ix_status, st.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    if (geom)
+      HIPCHK(hipMemcpyAsync(ctx->ix_dxi, dx.data(), sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  return OV_OK;
+}
+
+// Write the index-space state back into the caller's tangle: links and
+// status of every slot (new points get pin_wall NOT_A_FACE, as add_point
+// sets it), positions and, with geom, dxi / s' / s'' of the active points,
+// with vel also vels / vs.
+int ix_download(ov_ctx* ctx, ov_tangle_state* t, bool geom, bool vel) {
+  const int N = ctx->ix_N;
+  if (N == 0) return OV_OK;
+  std::vector<int> f(N), r(N), st(N);
+  HIPCHK(hipMemcpyAsync(f.data(), ctx->ix_fwd, sizeof(int) * N, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(r.data(), ctx->ix_rev, sizeof(int) * N, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(st.data(), ctx->ix_status, sizeof(int) * N, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int rc;
+  if ((rc = ix_down3(ctx, ctx->ix_X, t->vnodes, st.data()))) return rc;
+  if (geom) {
+    if ((rc = ix_down3(ctx, ctx->ix_T, t->tangents, st.data())) ||
+        (rc = ix_down3(ctx, ctx->ix_N3, t->normals, st.data())))
+      return rc;
+    std::vector<double> dx(N);
+    HIPCHK(hipMemcpyAsync(dx.data(), ctx->ix_dxi, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < N; ++k)
+      if (st[k] != OV_EMPTY) t->dxi[k] = dx[k];
+  }
+  if (vel) {
+    if ((rc = ix_down3(ctx, ctx->ix_vels, t->vels, st.data())) ||
+        (rc = ix_down3(ctx, ctx->ix_vs, t->vs, st.data())))
+      return rc;
+  }
+  for (int k = 0; k < N; ++k) {
+    t->connections[k].forward = f[k];
+    t->connections[k].reverse = r[k];
+    if (st[k] != OV_EMPTY && ctx->ix_status_in[k] == OV_EMPTY) t->status[k].pin_wall = -1;
+    t->status[k].status = st[k];
+  }
+  return OV_OK;
+}
+
+TopoArrays ix_arrays(ov_ctx* ctx) {
+  TopoArrays A;
+  A.X = ctx->ix_X;
+  A.T = ctx->ix_T;
+  A.Nn = ctx->ix_N3;
+  A.dxi = ctx->ix_dxi;
+  A.fwd = ctx->ix_fwd;
+  A.rev = ctx->ix_rev;
+  A.status = ctx->ix_status;
+  A.cap = ctx->ix_alloc;
+  A.N = ctx->ix_N;
+  return A;
+}
+
+// remesh on the index-space tangle; next_free in/out.  OV_E_NOMEM when the
+// reference would have to expand the tangle (no EMPTY slot left).
+int ix_remesh(ov_ctx* ctx, double min_dist, double max_dist, double max_curv, double dl_max,
+              int* next_free, int* added) {
+  int st[8] = {*next_free, 0, 0, 0, 0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(ctx->ix_state, st, sizeof st, hipMemcpyHostToDevice, ctx->stream));
+  int rc;
+  if ((rc = launch(ctx, "k_remesh", k_remesh, dim3(1), dim3(64), ix_arrays(ctx), ctx->box, min_dist,
+                   max_dist, max_curv, dl_max, ctx->ix_state)))
+    return rc;
+  // "if (added) enforce_boundaries" (tangle.c:870-872), decided on the device
+  if ((rc = launch(ctx, "k_wrap_index", k_wrap_index, grid1(ctx->ix_N), dim3(256), ctx->ix_X,
+                   (const int*)ctx->ix_status, ctx->ix_N, (long)ctx->ix_alloc, ctx->box,
+                   ctx->wall_periodic_mask, (const int*)(ctx->ix_state + 1))))
+    return rc;
+  HIPCHK(hipMemcpyAsync(st, ctx->ix_state, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *next_free = st[0];
+  if (added) *added = st[1];
+  if (st[2] == 1)
+    return fail(ctx, OV_E_NOMEM,
+                "remesh: no EMPTY slot left in a tangle of %d points (the reference would "
+                "expand_tangle to %d): pass a tangle with more capacity", ctx->ix_N, 2 * ctx->ix_N);
+  return OV_OK;
+}
+
+// Loop labels (lowest index of each loop) and ranks along the loop; leaves
+// the label in ix_lab[0] and the rank in ix_d[0].
+int ix_label_loops(ov_ctx* ctx) {
+  const int N = ctx->ix_N;
+  int rc;
+  int rounds = 1;
+  while ((1L << rounds) < (long)N + 1) ++rounds;
+  ++rounds;
+  if ((rc = launch(ctx, "k_loop_init", k_loop_init, grid1(N), dim3(256), (const int*)ctx->ix_fwd,
+                   (const int*)ctx->ix_status, N, ctx->ix_lab[0], ctx->ix_jmp[0])))
+    return rc;
+  for (int q = 0; q < rounds; ++q) {
+    const int a = q & 1, b = a ^ 1;
+    if ((rc = launch(ctx, "k_loop_min", k_loop_min, grid1(N), dim3(256), (const int*)ctx->ix_lab[a],
+                     (const int*)ctx->ix_jmp[a], N, ctx->ix_lab[b], ctx->ix_jmp[b])))
+      return rc;
+  }
+  if (rounds & 1)
+    HIPCHK(hipMemcpyAsync(ctx->ix_lab[0], ctx->ix_lab[1], sizeof(int) * N, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  if ((rc = launch(ctx, "k_rank_init", k_rank_init, grid1(N), dim3(256), (const int*)ctx->ix_rev,
+                   (const int*)ctx->ix_lab[0], N, ctx->ix_d[0], ctx->ix_jmp[0])))
+    return rc;
+  for (int q = 0; q < rounds; ++q) {
+    const int a = q & 1, b = a ^ 1;
+    if ((rc = launch(ctx, "k_rank_step", k_rank_step, grid1(N), dim3(256), (const int*)ctx->ix_d[a],
+                     (const int*)ctx->ix_jmp[a], N, ctx->ix_d[b], ctx->ix_jmp[b])))
+      return rc;
+  }
+  if (rounds & 1)
+    HIPCHK(hipMemcpyAsync(ctx->ix_d[0], ctx->ix_d[1], sizeof(int) * N, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  return launch(ctx, "k_loop_len", k_loop_len, grid1(N), dim3(256), (const int*)ctx->ix_rev,
+                (const int*)ctx->ix_lab[0], (const int*)ctx->ix_d[0], N, ctx->ix_len);
+}
+
+int ix_eliminate(ov_ctx* ctx, const ov_topology* tp) {
+  const int N = ctx->ix_N;
+  if (N == 0) return OV_OK;
+  int rc;
+  if ((rc = ix_label_loops(ctx))) return rc;
+  const int mask = (tp->eliminate_origin_loops ? 1 : 0) | (tp->eliminate_zaxis_loops ? 2 : 0) |
+                   (tp->eliminate_outer_loops ? 4 : 0);
+  if ((rc = launch(ctx, "k_loop_region", k_loop_region, grid1(N), dim3(256), (const double*)ctx->ix_X,
+                   (const int*)ctx->ix_lab[0], N, (long)ctx->ix_alloc,
+                   tp->eliminate_loops_origin_cutoff, tp->eliminate_loops_zaxis_cutoff,
+                   tp->eliminate_outer_loops_cutoff, ctx->ix_bits)) ||
+      (rc = launch(ctx, "k_fill_int", k_fill_int, grid1(N), dim3(256), ctx->ix_all, N, 7)) ||
+      (rc = launch(ctx, "k_loop_and", k_loop_and, grid1(N), dim3(256), (const int*)ctx->ix_lab[0],
+                   (const int*)ctx->ix_bits, N, ctx->ix_all)))
+    return rc;
+  return launch(ctx, "k_loop_delete", k_loop_delete, grid1(N), dim3(256), (const int*)ctx->ix_lab[0],
+                (const int*)ctx->ix_len, (const int*)ctx->ix_all, N, mask, tp->small_loop_cutoff,
+                ctx->ix_fwd, ctx->ix_rev, ctx->ix_status);
+}
+
+// Stream order of the index-space tangle (loops in order of their lowest
+// index, each walked forward from it, as prepare() orders them): fills
+// ctx->n, the stream buffers' positions X0 and links fwd/rev, and
+// ix_act/ix_pos.
+int ix_to_stream(ov_ctx* ctx) {
+  const int N = ctx->ix_N;
+  int rc;
+  if ((rc = ix_label_loops(ctx))) return rc;
+  size_t sb = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, ctx->ix_len, ctx->ix_off, N, ctx->stream));
+  if ((rc = cub_temp(ctx, sb))) return rc;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->t_cub, sb, ctx->ix_len, ctx->ix_off, N, ctx->stream));
+  int last[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(&last[0], ctx->ix_off + (N - 1), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(&last[1], ctx->ix_len + (N - 1), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int n = last[0] + last[1];
+  ctx->n = n;
+  if ((rc = ensure_capacity(ctx, std::max(n, 1)))) return rc;
+  ctx->cap = ctx->buf_cap;
+  if ((rc = launch(ctx, "k_stream_pos", k_stream_pos, grid1(N), dim3(256), (const int*)ctx->ix_lab[0],
+                   (const int*)ctx->ix_d[0], (const int*)ctx->ix_off, N, ctx->ix_pos, ctx->ix_act)))
+    return rc;
+  if (n == 0) return OV_OK;
+  return launch(ctx, "k_gather_stream", k_gather_stream, grid1(n), dim3(256), (const double*)ctx->ix_X,
+                (long)ctx->ix_alloc, (const int*)ctx->ix_fwd, (const int*)ctx->ix_rev,
+                (const int*)ctx->ix_act, (const int*)ctx->ix_pos, n, ctx->cap, ctx->X0, ctx->fwd,
+                ctx->rev);
+}
+
+int ix_scatter3(ov_ctx* ctx, const double* src, double* dst) {
+  if (ctx->n == 0) return OV_OK;
+  return launch(ctx, "k_scatter3", k_scatter3, grid1(ctx->n), dim3(256), src, ctx->cap,
+                (const int*)ctx->ix_act, ctx->n, (long)ctx->ix_alloc, dst);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1393,6 +1690,12 @@ void ov_ctx_destroy(ov_ctx* ctx) {
   void* rb[] = {ctx->r_keys, ctx->r_keys_sorted, ctx->r_vals,  ctx->r_order,
                 ctx->r_act,  ctx->r_count,       ctx->r_pairs, ctx->r_pairs_sorted};
   for (void* b : rb)
+    if (b) hipFree(b);
+  void* ib[] = {ctx->ix_X, ctx->ix_T, ctx->ix_N3, ctx->ix_dxi, ctx->ix_vels, ctx->ix_vs, ctx->ix_fwd,
+                ctx->ix_rev, ctx->ix_status, ctx->ix_lab[0], ctx->ix_lab[1], ctx->ix_jmp[0],
+                ctx->ix_jmp[1], ctx->ix_d[0], ctx->ix_d[1], ctx->ix_len, ctx->ix_off, ctx->ix_pos,
+                ctx->ix_act, ctx->ix_bits, ctx->ix_all, ctx->ix_state};
+  for (void* b : ib)
     if (b) hipFree(b);
   if (ctx->hst) hipHostFree(ctx->hst);
   for (auto& p : ctx->pending) {
@@ -2105,6 +2408,120 @@ int ov_gpu_velocities_noalloc(ov_ctx* ctx, const double* pos, const int* fwd, co
   if (hipEventElapsedTime(&ms, ctx->t_begin, ctx->t_end) == hipSuccess) ctx->last_device_ms = ms;
   ctx->last_interactions = v.last_interactions;
   if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite velocity");
+  return OV_OK;
+}
+
+void ov_topology_default(ov_topology* tp) {
+  if (!tp) return;
+  tp->dl_min = 1e-3;
+  tp->dl_max = 5e-3;
+  tp->small_loop_cutoff = 5;
+  tp->eliminate_origin_loops = 0;
+  tp->eliminate_loops_origin_cutoff = 3e-2;
+  tp->eliminate_zaxis_loops = 0;
+  tp->eliminate_loops_zaxis_cutoff = 2e-2;
+  tp->eliminate_outer_loops = 0;
+  tp->eliminate_outer_loops_cutoff = 1;
+}
+
+int ov_gpu_remesh(ov_ctx* ctx, ov_tangle_state* t, const ov_params* p, double min_dist,
+                  double max_dist) {
+  if (!ctx) return OV_E_INVALID;
+  if (!p) return fail(ctx, OV_E_INVALID, "null params");
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->profile = false;
+  int rc = ix_prepare(ctx, t, p);
+  if (rc || ctx->ix_N == 0) return rc;
+  if ((rc = ix_upload(ctx, t, true, false))) return rc;
+  int nf = t->next_free;
+  if ((rc = ix_remesh(ctx, min_dist, max_dist, p->max_curvature_scale, p->dl_max, &nf, nullptr)))
+    return rc;
+  if ((rc = ix_download(ctx, t, true, false))) return rc;
+  t->next_free = nf;
+  return OV_OK;
+}
+
+int ov_gpu_eliminate_small_loops(ov_ctx* ctx, ov_tangle_state* t, const ov_topology* tp) {
+  if (!ctx) return OV_E_INVALID;
+  if (!tp) return fail(ctx, OV_E_INVALID, "null topology settings");
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->profile = false;
+  int rc = ix_prepare(ctx, t, nullptr);
+  if (rc || ctx->ix_N == 0) return rc;
+  if ((rc = ix_upload(ctx, t, false, false)) || (rc = ix_eliminate(ctx, tp))) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return ix_download(ctx, t, false, false);
+}
+
+int ov_gpu_enforce_boundaries(ov_ctx* ctx, ov_tangle_state* t) {
+  if (!ctx) return OV_E_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->profile = false;
+  int rc = ix_prepare(ctx, t, nullptr);
+  if (rc || ctx->ix_N == 0) return rc;
+  if ((rc = ix_upload(ctx, t, false, false))) return rc;
+  if ((rc = launch(ctx, "k_wrap_index", k_wrap_index, grid1(ctx->ix_N), dim3(256), ctx->ix_X,
+                   (const int*)ctx->ix_status, ctx->ix_N, (long)ctx->ix_alloc, ctx->box,
+                   ctx->wall_periodic_mask, (const int*)nullptr)))
+    return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return ix_download(ctx, t, false, false);
+}
+
+int ov_gpu_run(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps, const ov_params* p,
+               const ov_topology* tp, int flags) {
+  if (!ctx) return OV_E_INVALID;
+  if (!p || !tp) return fail(ctx, OV_E_INVALID, "null params or topology settings");
+  if (nsteps < 0) return fail(ctx, OV_E_INVALID, "negative step count");
+  if (flags & OV_USE_CALLER_K1) return fail(ctx, OV_E_INVALID, "ov_gpu_run evaluates k1 itself");
+  if (p->use_BH) return fail(ctx, OV_E_UNSUPPORTED, "ov_gpu_run: direct sum only (use_BH = 0)");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = ix_prepare(ctx, t, p);
+  if (rc) return rc;
+  if (ctx->ix_N == 0 || nsteps == 0) return OV_OK;
+  if (!t->vels || !t->vs || !t->tangents || !t->normals || !t->dxi)
+    return fail(ctx, OV_E_INVALID, "null tangle arrays");
+  ctx->profile = false;
+  if (!ctx->img && (rc = dalloc(ctx, (void**)&ctx->img, sizeof(ImgDev) * MAX_IMAGES))) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->img, ctx->imgs.data(), sizeof(ImgDev) * ctx->imgs.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = ix_upload(ctx, t, false, true))) return rc;
+  ctx->profile = (flags & OV_PROFILE) != 0;
+  if ((rc = launch(ctx, "k_zero_int", k_zero_int, dim3(1), dim3(64), ctx->flag))) return rc;
+  if ((rc = mark_begin(ctx))) return rc;
+  double time = t0;
+  int nf = t->next_free;
+  for (int step = 0; step < nsteps; ++step) {
+    // main.c:112-117: update_tangle; rk4_step; update_tangents_normals;
+    // remesh; eliminate_small_loops; enforce_boundaries
+    if ((rc = ix_to_stream(ctx))) return rc;
+    if (ctx->n == 0) break;
+    if ((rc = rk4_device(ctx, time, dt, p->rk_variant, false))) return rc;
+    if ((rc = ix_scatter3(ctx, ctx->K[0], ctx->ix_vels)) || (rc = ix_scatter3(ctx, ctx->vs1, ctx->ix_vs)))
+      return rc;
+    if ((rc = launch(ctx, "k_geometry", k_geometry, grid1(ctx->n), dim3(256), (const double*)ctx->X0,
+                     (const int*)ctx->fwd, (const int*)ctx->rev, ctx->n, ctx->cap, ctx->box, ctx->ph, 1,
+                     ctx->T, ctx->Nn, ctx->dxi, ctx->lia)))
+      return rc;
+    if ((rc = ix_scatter3(ctx, ctx->X0, ctx->ix_X)) || (rc = ix_scatter3(ctx, ctx->T, ctx->ix_T)) ||
+        (rc = ix_scatter3(ctx, ctx->Nn, ctx->ix_N3)) ||
+        (rc = launch(ctx, "k_scatter1", k_scatter1, grid1(ctx->n), dim3(256), (const double*)ctx->dxi,
+                     (const int*)ctx->ix_act, ctx->n, ctx->ix_dxi)))
+      return rc;
+    if ((rc = ix_remesh(ctx, tp->dl_min, tp->dl_max, p->max_curvature_scale, p->dl_max, &nf, nullptr)))
+      return rc;
+    if ((rc = ix_eliminate(ctx, tp))) return rc;
+    if ((rc = launch(ctx, "k_wrap_index", k_wrap_index, grid1(ctx->ix_N), dim3(256), ctx->ix_X,
+                     (const int*)ctx->ix_status, ctx->ix_N, (long)ctx->ix_alloc, ctx->box,
+                     ctx->wall_periodic_mask, (const int*)nullptr)))
+      return rc;
+    time += dt;
+  }
+  int flag = 0;
+  if ((rc = finish(ctx, &flag))) return rc;
+  if ((rc = ix_download(ctx, t, true, true))) return rc;
+  t->next_free = nf;
+  if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite %s", (flag & 2) ? "position" : "velocity");
   return OV_OK;
 }
 

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .tangle import Params, Tangle, TangleState
+from .tangle import Params, Tangle, TangleState, Topology
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libopenvort_gpu.so")
@@ -40,6 +40,11 @@ SYMBOLS = (
     "ov_gpu_rk4_run_vjp",
     "ov_gpu_reconnection_candidates",
     "ov_gpu_velocities_noalloc",
+    "ov_topology_default",
+    "ov_gpu_remesh",
+    "ov_gpu_eliminate_small_loops",
+    "ov_gpu_enforce_boundaries",
+    "ov_gpu_run",
 )
 
 _lib = None
@@ -108,6 +113,17 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
     lib.ov_gpu_velocities_noalloc.restype = C.c_int
+    lib.ov_topology_default.argtypes = [P(Topology)]
+    lib.ov_topology_default.restype = None
+    lib.ov_gpu_remesh.argtypes = [C.c_void_p, P(TangleState), P(Params), C.c_double, C.c_double]
+    lib.ov_gpu_remesh.restype = C.c_int
+    lib.ov_gpu_eliminate_small_loops.argtypes = [C.c_void_p, P(TangleState), P(Topology)]
+    lib.ov_gpu_eliminate_small_loops.restype = C.c_int
+    lib.ov_gpu_enforce_boundaries.argtypes = [C.c_void_p, P(TangleState)]
+    lib.ov_gpu_enforce_boundaries.restype = C.c_int
+    lib.ov_gpu_run.argtypes = [C.c_void_p, P(TangleState), C.c_double, C.c_double, C.c_int, P(Params),
+                               P(Topology), C.c_int]
+    lib.ov_gpu_run.restype = C.c_int
     _lib = lib
     return lib
 
@@ -232,6 +248,31 @@ class Context:
                                                 C.addressof(ts.box), C.addressof(ts.bimg), time,
                                                 C.byref(params), tangents, normals, dxi, vs, vels)
         return self._check(rc)
+
+    def remesh(self, t: Tangle, params: Params, min_dist: float, max_dist: float) -> int:
+        """remesh (tangle.c:824-873) on the GPU; t's geometry must be current."""
+        ts = t.cstruct()
+        rc = self._check(self.lib.ov_gpu_remesh(self.ptr, C.byref(ts), C.byref(params), min_dist, max_dist))
+        t.next_free = ts.next_free
+        return rc
+
+    def eliminate_small_loops(self, t: Tangle, tp: Topology) -> int:
+        ts = t.cstruct()
+        return self._check(self.lib.ov_gpu_eliminate_small_loops(self.ptr, C.byref(ts), C.byref(tp)))
+
+    def enforce_boundaries(self, t: Tangle) -> int:
+        ts = t.cstruct()
+        return self._check(self.lib.ov_gpu_enforce_boundaries(self.ptr, C.byref(ts)))
+
+    def run(self, t: Tangle, t0: float, dt: float, nsteps: int, params: Params, tp: Topology,
+            profile=False, allow_nonfinite=False) -> int:
+        """nsteps of the reference main loop body (main.c:112-117) device-resident."""
+        ts = t.cstruct()
+        rc = self.lib.ov_gpu_run(self.ptr, C.byref(ts), t0, dt, nsteps, C.byref(params), C.byref(tp),
+                                 OV_PROFILE if profile else 0)
+        if rc in (OV_OK, OV_E_NONFINITE):
+            t.next_free = ts.next_free
+        return self._check(rc, allow_nonfinite)
 
     def batch_tuning(self):
         u, b, ms = C.c_int(), C.c_int(), C.c_double()

@@ -123,6 +123,25 @@ def set_flow(f: Flow, name: str = "no flow", **params) -> Flow:
             f.a[scalar] = float(params.get(pn, 0.0))
             scalar += 1
     return f
+class Topology(C.Structure):
+    """ov_topology: settings of remesh and eliminate_small_loops (reference
+    globals, vortex_constants.c:39-66)."""
+    _fields_ = [
+        ("dl_min", C.c_double), ("dl_max", C.c_double), ("small_loop_cutoff", C.c_int),
+        ("eliminate_origin_loops", C.c_int), ("eliminate_loops_origin_cutoff", C.c_double),
+        ("eliminate_zaxis_loops", C.c_int), ("eliminate_loops_zaxis_cutoff", C.c_double),
+        ("eliminate_outer_loops", C.c_int), ("eliminate_outer_loops_cutoff", C.c_double),
+    ]
+
+
+def default_topology(**kw) -> Topology:
+    """Reference defaults (vortex_constants.c:39-66)."""
+    tp = Topology(1e-3, 5e-3, 5, 0, 3e-2, 0, 2e-2, 0, 1.0)
+    for k, v in kw.items():
+        setattr(tp, k, v)
+    return tp
+
+
 PREC_FP64, PREC_MIXED = 0, 1
 RK_REFERENCE, RK_CLASSIC = 0, 1
 MP_REFERENCE, MP_DIPOLE = 0, 1
@@ -260,6 +279,7 @@ class Tangle:
     tangents: np.ndarray = None
     normals: np.ndarray = None
     dxi: np.ndarray = None
+    next_free: int = None      # the reference's next_free (None: N)
 
     def __post_init__(self):
         N = self.N
@@ -285,7 +305,7 @@ class Tangle:
     def copy(self) -> "Tangle":
         t = Tangle(self.vnodes.copy(), self.connections.copy(), self.status.copy(), self.box_lo,
                    self.box_hi, self.walls, list(self.images), self.vels.copy(), self.vs.copy(),
-                   self.tangents.copy(), self.normals.copy(), self.dxi.copy())
+                   self.tangents.copy(), self.normals.copy(), self.dxi.copy(), self.next_free)
         return t
 
     def cstruct(self) -> TangleState:
@@ -314,7 +334,7 @@ class Tangle:
         ts.bimg.images = C.cast(self._img, C.POINTER(ImageTangle)) if n else None
         ts.bimg.n = n
         ts.N = self.N
-        ts.next_free = self.N
+        ts.next_free = self.N if self.next_free is None else self.next_free
         ts.total_free = 0
         return ts
 

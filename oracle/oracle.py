@@ -186,6 +186,11 @@ class Reference:
             lib.ovref_vconf_size.restype = C.c_int
             lib.ovref_vconf.argtypes = [P(Flow), C.c_void_p]
             lib.ovref_vconf.restype = C.c_int
+            lib.ovref_remesh.argtypes = [P(TangleState), C.c_double, C.c_double, C.c_int]
+            lib.ovref_remesh.restype = C.c_int
+            lib.ovref_eliminate_small_loops.argtypes = [P(TangleState), C.c_int, C.c_int, C.c_double,
+                                                        C.c_int, C.c_double, C.c_int, C.c_double]
+            lib.ovref_eliminate_small_loops.restype = None
             Reference._lib = lib
         self.lib = Reference._lib
         self.threads = threads
@@ -272,3 +277,18 @@ class Reference:
         if self.lib.ovref_vconf(C.byref(f), buf) != 0:
             raise RuntimeError("unknown flow")
         return buf
+
+    def remesh(self, t: Tangle, min_dist: float, max_dist: float, need_free: int = 1):
+        """The reference's remesh (globals from set_params: dl_max and
+        max_curvature_scale of add_point); t.next_free is updated."""
+        ts = t.cstruct()
+        if self.lib.ovref_remesh(C.byref(ts), min_dist, max_dist, need_free) != 0:
+            raise RuntimeError("not enough EMPTY slots for the reference's remesh")
+        t.next_free = ts.next_free
+
+    def eliminate_small_loops(self, t: Tangle, loop_length: int, origin=(0, 3e-2), zaxis=(0, 2e-2),
+                              outer=(0, 1.0)):
+        ts = t.cstruct()
+        self.lib.ovref_eliminate_small_loops(C.byref(ts), loop_length, origin[0], origin[1], zaxis[0],
+                                             zaxis[1], outer[0], outer[1])
+

@@ -244,3 +244,35 @@ int ovref_vconf(const ov_flow *f, void *out) {
   memcpy(out, &c, sizeof c);
   return 0;
 }
+
+/* ---- topology edits (tangle.c:796-1070) ------------------------------------ */
+
+extern double global_dl_min;
+extern int global_eliminate_origin_loops, global_eliminate_zaxis_loops, global_eliminate_outer_loops;
+extern double global_eliminate_loops_origin_cutoff, global_eliminate_loops_zaxis_cutoff,
+    global_eliminate_outer_loops_cutoff;
+
+/* The reference's remesh (tangle.c:824-873).  The caller's tangle must have
+ * EMPTY slots for every point remesh adds (expand_tangle would realloc the
+ * caller's arrays); returns -1 without calling remesh when fewer than
+ * `need_free` slots are EMPTY. */
+int ovref_remesh(ov_tangle_state *t, double min_dist, double max_dist, int need_free) {
+  int free_slots = 0;
+  for (int k = 0; k < t->N; ++k) free_slots += t->status[k].status == OV_EMPTY;
+  if (free_slots < need_free) return -1;
+  remesh((struct tangle_state *)t, min_dist, max_dist);
+  return 0;
+}
+
+/* The reference's eliminate_small_loops (tangle.c:878-945) with the
+ * elimination globals set first. */
+void ovref_eliminate_small_loops(ov_tangle_state *t, int loop_length, int origin, double origin_cut,
+                                 int zaxis, double zaxis_cut, int outer, double outer_cut) {
+  global_eliminate_origin_loops = origin;
+  global_eliminate_loops_origin_cutoff = origin_cut;
+  global_eliminate_zaxis_loops = zaxis;
+  global_eliminate_loops_zaxis_cutoff = zaxis_cut;
+  global_eliminate_outer_loops = outer;
+  global_eliminate_outer_loops_cutoff = outer_cut;
+  eliminate_small_loops((struct tangle_state *)t, loop_length);
+}
