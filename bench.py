@@ -219,6 +219,8 @@ def main():
                     help="Kelvin-ring gradient (4096 points, 200 steps, checkpoints every 20; 0: skip)")
     ap.add_argument("--reconnect", type=int, default=1,
                     help="reconnection candidate search on the workload (0: skip)")
+    ap.add_argument("--topology", type=int, default=1,
+                    help="device-resident topology edits and the 200-step main loop (0: skip)")
     ap.add_argument("--tree", type=int, default=1,
                     help="octree evaluations at theta 0.3/0.5/0.7 against the FP64 direct sum (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -470,6 +472,12 @@ def main():
     elif args.reconnect > 0:
         skipped.append("reconnection_candidates")
 
+    if args.topology > 0 and left() > 45:
+        out["topology"] = topology_section(ctx, T, tg, eng if kind == "reference" else None, threads)
+        log(f"topology: {out['topology']}")
+    elif args.topology > 0:
+        skipped.append("topology")
+
     if args.ensemble > 0 and left() > 60:
         out["ensemble"] = ensemble_section(ctx, T, args)
         log(f"ensemble: {out['ensemble']}")
@@ -487,6 +495,108 @@ def main():
     out["wall_s"] = round(time.time() - t_start, 1)
     res.update(out)
     print(json.dumps(res), flush=True)
+
+
+TOPO_KERNELS = ("k_remesh", "k_wrap_index", "k_loop_init", "k_loop_min", "k_rank_init", "k_rank_step",
+                "k_loop_len", "k_loop_region", "k_fill_int", "k_loop_and", "k_loop_delete",
+                "k_stream_pos", "k_gather_stream", "k_scatter3", "k_scatter1")
+
+
+def jittered(t, spacing, amp, seed, capacity):
+    """t's points each moved by a seeded uniform offset in [-amp, amp] x
+    spacing per axis (segment lengths spread around the spacing), in a copy
+    of the given capacity."""
+    x = t.expand(capacity)
+    act = x.active
+    x.vnodes[act] += np.random.default_rng(seed).uniform(-amp, amp, (len(act), 3)) * spacing
+    return x
+
+
+def topology_section(ctx, T, tg, ref, threads):
+    """Device-resident topology edits.  (a) The 1M-point tangle, each point
+    moved by a seeded offset of up to 0.15 x spacing per axis, 15 % spare
+    capacity: remesh with dl_min / dl_max = 0.6 / 1.2 x spacing (the example
+    configuration's 1:2 ratio; ~6 % of the segments are too long),
+    eliminate_small_loops, enforce_boundaries.  (b) The 64-ring periodic
+    tangle (0.1 cm box, 2e-4 cm spacing) moved the same way: remesh on the
+    GPU and with the reference's own remesh on the host (its OpenMP loops on
+    `threads` threads), links compared.  (c) The reference's main-loop body
+    (update_tangle, rk4_step, update_tangents_normals, remesh,
+    eliminate_small_loops, enforce_boundaries) for 200 steps of the 64-ring
+    tangle with v_n = 0.5 cm/s along x and dt = 5e-6 s, dl_min / dl_max =
+    1.2e-4 / 2.4e-4 cm, mixed direct sum, resident on the device
+    (ov_gpu_run)."""
+    res = {}
+    x = jittered(tg, 1e-4, 0.15, 5, tg.N + (tg.N * 3) // 20)
+    ctx.update_tangents_normals(x)
+    lo, hi = 0.6e-4, 1.2e-4
+    p = T.default_params(dl_max=hi)
+    n0 = len(x.active)
+    w0 = time.time()
+    ctx.remesh(x, p, lo, hi)
+    wall = (time.time() - w0) * 1e3
+    rm = ctx.last_device_ms()
+    n1 = len(x.active)
+    inserted = int(np.count_nonzero(x.status[tg.N:, 0] != T.EMPTY))
+    w0 = time.time()
+    ctx.eliminate_small_loops(x, T.default_topology())
+    el_wall, el = (time.time() - w0) * 1e3, ctx.last_device_ms()
+    w0 = time.time()
+    ctx.enforce_boundaries(x)
+    en_wall, en = (time.time() - w0) * 1e3, ctx.last_device_ms()
+    res["remesh_1m"] = {"points": n0, "capacity": x.N, "dl_min_cm": lo, "dl_max_cm": hi,
+                        "points_after": n1, "inserted_above_N": inserted,
+                        "device_ms": round(rm, 2), "wall_ms": round(wall, 1),
+                        "eliminate_small_loops_device_ms": round(el, 3), "eliminate_wall_ms": round(el_wall, 1),
+                        "enforce_boundaries_device_ms": round(en, 3), "enforce_wall_ms": round(en_wall, 1)}
+
+    base = T.random_rings(1001, 0.1, n_rings=64)
+    c1 = jittered(base, 2e-4, 0.15, 7, 2 * base.N)
+    ctx.update_tangents_normals(c1)
+    lo1, hi1 = 1.2e-4, 2.4e-4
+    p1 = T.default_params(dl_max=hi1)
+    g = c1.copy()
+    ctx.remesh(g, p1, lo1, hi1)
+    r1 = {"points": int(len(c1.active)), "dl_min_cm": lo1, "dl_max_cm": hi1,
+          "points_after": int(len(g.active)), "gpu_device_ms": round(ctx.last_device_ms(), 3)}
+    if ref is not None:
+        # the reference's own remesh (oracle/_ref) on the same tangle
+        a = c1.copy()
+        ref.set_params(p1)
+        ref.update_tangents_normals(a)
+        c0 = time.time()
+        ref.remesh(a, lo1, hi1, need_free=len(c1.active))
+        r1["cpu_reference_ms"] = round((time.time() - c0) * 1e3, 2)
+        r1["cpu_threads"] = threads
+        r1["links_identical"] = bool(np.array_equal(a.connections, g.connections)
+                                     and np.array_equal(a.status[:, 0], g.status[:, 0]))
+        r1["max_position_diff_cm"] = float(np.abs(a.vnodes[a.active] - g.vnodes[a.active]).max())
+    res["remesh_64_rings"] = r1
+
+    c2 = T.random_rings(1001, 0.1, n_rings=64)
+    c2 = c2.expand(2 * c2.N)
+    p2 = T.default_params(vortex_width=1.3e-8, precision=T.PREC_MIXED, near_radius=0.01, dl_max=2.4e-4,
+                          vn=(0.5, 0.0, 0.0))
+    tp = T.default_topology(dl_min=1.2e-4, dl_max=2.4e-4)
+    ctx.update_tangents_normals(c2)
+    n_start = len(c2.active)
+    w0 = time.time()
+    ctx.run(c2, 0.0, 5e-6, 200, p2, tp)
+    run_wall = time.time() - w0
+    run_ms = ctx.last_device_ms()
+    n_end = int(len(c2.active))
+    # 20 more steps with per-kernel timing, for the breakdown
+    ctx.reset_stats()
+    ctx.run(c2, 200 * 5e-6, 5e-6, 20, p2, tp, profile=True)
+    topo_ms = sum(ctx.kernel_stats(k)[1] for k in TOPO_KERNELS)
+    res["main_loop_64_rings"] = {
+        "steps": 200, "dt_s": 5e-6, "points_start": n_start, "points_end": n_end,
+        "device_ms_per_step": round(run_ms / 200, 3), "wall_s": round(run_wall, 2),
+        "profiled_steps": 20,
+        "topology_kernels_ms_per_step": round(topo_ms / 20, 3),
+        "k_remesh_ms_per_step": round(ctx.kernel_stats("k_remesh")[1] / 20, 3),
+        "k_direct_mixed_ms_per_step": round(ctx.kernel_stats("k_direct_mixed")[1] / 20, 3)}
+    return res
 
 
 def ensemble_section(ctx, T, args):

@@ -347,8 +347,9 @@ void ov_ctx_reset_stats(ov_ctx *ctx);
  * direct-sum count N_tgt (N_seg - 2) + n_img N_tgt N_seg). */
 double ov_ctx_last_interactions(const ov_ctx *ctx);
 
-/* Device time of the last velocities / rk4 call in milliseconds: HIP events
- * on the context stream around the kernels, host<->device copies excluded. */
+/* Device time of the last velocities / rk4 / run / topology-edit call in
+ * milliseconds: HIP events on the context stream around the kernels,
+ * host<->device copies of the tangle excluded. */
 double ov_ctx_last_device_ms(const ov_ctx *ctx);
 
 /* Octree of the last evaluation with use_BH: cell count; leaf segment-point

@@ -964,6 +964,15 @@ int mark_begin(ov_ctx* ctx) {
   return OV_OK;
 }
 
+// Device time since mark_begin, into last_device_ms (no flag read).
+int mark_end(ov_ctx* ctx) {
+  HIPCHK(hipEventRecord(ctx->t_end, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, ctx->t_begin, ctx->t_end) == hipSuccess) ctx->last_device_ms = ms;
+  return OV_OK;
+}
+
 int finish(ov_ctx* ctx, int* flag_out) {
   HIPCHK(hipEventRecord(ctx->t_end, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -2432,9 +2441,10 @@ int ov_gpu_remesh(ov_ctx* ctx, ov_tangle_state* t, const ov_params* p, double mi
   ctx->profile = false;
   int rc = ix_prepare(ctx, t, p);
   if (rc || ctx->ix_N == 0) return rc;
-  if ((rc = ix_upload(ctx, t, true, false))) return rc;
+  if ((rc = ix_upload(ctx, t, true, false)) || (rc = mark_begin(ctx))) return rc;
   int nf = t->next_free;
-  if ((rc = ix_remesh(ctx, min_dist, max_dist, p->max_curvature_scale, p->dl_max, &nf, nullptr)))
+  if ((rc = ix_remesh(ctx, min_dist, max_dist, p->max_curvature_scale, p->dl_max, &nf, nullptr)) ||
+      (rc = mark_end(ctx)))
     return rc;
   if ((rc = ix_download(ctx, t, true, false))) return rc;
   t->next_free = nf;
@@ -2448,8 +2458,9 @@ int ov_gpu_eliminate_small_loops(ov_ctx* ctx, ov_tangle_state* t, const ov_topol
   ctx->profile = false;
   int rc = ix_prepare(ctx, t, nullptr);
   if (rc || ctx->ix_N == 0) return rc;
-  if ((rc = ix_upload(ctx, t, false, false)) || (rc = ix_eliminate(ctx, tp))) return rc;
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if ((rc = ix_upload(ctx, t, false, false)) || (rc = mark_begin(ctx)) || (rc = ix_eliminate(ctx, tp)) ||
+      (rc = mark_end(ctx)))
+    return rc;
   return ix_download(ctx, t, false, false);
 }
 
@@ -2459,12 +2470,12 @@ int ov_gpu_enforce_boundaries(ov_ctx* ctx, ov_tangle_state* t) {
   ctx->profile = false;
   int rc = ix_prepare(ctx, t, nullptr);
   if (rc || ctx->ix_N == 0) return rc;
-  if ((rc = ix_upload(ctx, t, false, false))) return rc;
+  if ((rc = ix_upload(ctx, t, false, false)) || (rc = mark_begin(ctx))) return rc;
   if ((rc = launch(ctx, "k_wrap_index", k_wrap_index, grid1(ctx->ix_N), dim3(256), ctx->ix_X,
                    (const int*)ctx->ix_status, ctx->ix_N, (long)ctx->ix_alloc, ctx->box,
-                   ctx->wall_periodic_mask, (const int*)nullptr)))
+                   ctx->wall_periodic_mask, (const int*)nullptr)) ||
+      (rc = mark_end(ctx)))
     return rc;
-  HIPCHK(hipStreamSynchronize(ctx->stream));
   return ix_download(ctx, t, false, false);
 }
 

@@ -308,6 +308,25 @@ class Tangle:
                    self.tangents.copy(), self.normals.copy(), self.dxi.copy(), self.next_free)
         return t
 
+    def expand(self, new_N: int) -> "Tangle":
+        """A copy with capacity new_N: the points keep their indices and the
+        slots above N are EMPTY (the reference's expand_tangle, tangle.c:
+        114-150); next_free stays, or becomes N when it was unset."""
+        N = self.N
+        if new_N < N:
+            raise ValueError(f"expand to {new_N} below the capacity {N}")
+        k = new_N - N
+        st = np.vstack([self.status, np.tile(np.array([[EMPTY, -1]], dtype=np.int32), (k, 1))])
+        t = Tangle(np.vstack([self.vnodes, np.zeros((k, 3))]),
+                   np.vstack([self.connections, np.full((k, 2), -1, dtype=np.int32)]), st,
+                   self.box_lo, self.box_hi, self.walls, list(self.images),
+                   np.vstack([self.vels, np.zeros((k, 3))]), np.vstack([self.vs, np.zeros((k, 3))]),
+                   np.vstack([self.tangents, np.zeros((k, 3))]),
+                   np.vstack([self.normals, np.zeros((k, 3))]),
+                   np.concatenate([self.dxi, np.zeros(k)]),
+                   N if self.next_free is None else self.next_free)
+        return t
+
     def cstruct(self) -> TangleState:
         """ov_tangle_state view of the arrays (valid while self lives)."""
         ts = TangleState()
