@@ -112,6 +112,7 @@ struct ov_ctx {
   void* t_cub = nullptr;
   size_t t_cub_bytes = 0;
   long* t_stats = nullptr;
+  int oct_images_per_walk = 9;   // most images per traversal wavefront (64 / 9 = 7 targets)
   int* t_big = nullptr;
   long t_big_cap = 0;
   double* t_big_part = nullptr;
@@ -777,39 +778,46 @@ int tree_sum(ov_ctx* ctx, const double* X) {
   const long cap = ctx->cap;
   int rc;
   if ((rc = build_tree(ctx, X))) return rc;
-  const int n_tgt = (n + TILE - 1) / TILE;
   const int n_img = (int)ctx->imgs.size();
+  // images per wavefront: at most oct_images_per_walk, balanced over the
+  // groups so that the last group's wavefronts are as full as the others
+  const int cap_walk = std::min(std::max(ctx->oct_images_per_walk, 1), n_img);
+  const int n_groups = (n_img + cap_walk - 1) / cap_walk;
+  const int per_walk = (n_img + n_groups - 1) / n_groups;
+  const int tpw = TILE / per_walk;                 // targets per wavefront
   size_t pbytes = sizeof(double) * 3 * (size_t)cap * (size_t)n_img;
   if (pbytes > ctx->partial_bytes) {
     if ((rc = dalloc(ctx, (void**)&ctx->partial, pbytes))) return rc;
     ctx->partial_bytes = pbytes;
   }
-  if ((rc = launch(ctx, "k_tgt_tiles", k_tgt_tiles, dim3(n_tgt), dim3(TILE), X, n, cap, ctx->tgt)))
-    return rc;
   HIPCHK(hipMemsetAsync(ctx->t_stats, 0, 3 * sizeof(long), ctx->stream));
   OctArgs A;
   A.P = X;
   A.cap = cap;
   A.n = n;
-  A.n_tgt_tiles = n_tgt;
   A.nodes = ctx->t_nodes;
   A.nmom = ctx->t_nmom;
   A.n_nodes = (int)ctx->tree_cells;
   A.segs = ctx->t_segs;
   A.img = ctx->img;
   A.n_img = n_img;
+  A.img_per_walk = per_walk;
+  A.n_groups = n_groups;
+  const long n_waves = (n + tpw - 1) / tpw;
+  A.n_items = (int)((n_waves + OCT_WAVES - 1) / OCT_WAVES) * n_groups;
+  A.per_xcd = (A.n_items + 7) / 8;
   A.op = ctx->op;
   A.kappa4pi = ctx->ph.kappa / 4 / 3.14159265358979323846;
   A.partial = ctx->partial;
   A.stats = ctx->profile ? ctx->t_stats : nullptr;
-  const dim3 grid((unsigned)((n_tgt + 3) / 4), (unsigned)n_img);
+  const dim3 grid((unsigned)(8 * A.per_xcd));
   if (ctx->mixed)
     rc = launch(ctx, "k_octree_mixed", k_octree_mixed, grid, dim3(256), A);
   else
     rc = launch(ctx, "k_octree_fp64", k_octree_fp64, grid, dim3(256), A);
   if (rc) return rc;
-  if ((rc = launch(ctx, "k_reduce", k_reduce, dim3(grid1(n).x, 3), dim3(256), (const double*)ctx->partial, n_img, n,
-                   cap, ctx->bs)))
+  if ((rc = launch(ctx, "k_reduce", k_reduce, dim3(grid1(n).x, 3), dim3(256), (const double*)ctx->partial, n_img,
+                   n, cap, ctx->bs)))
     return rc;
   if (ctx->profile) {
     long st[3] = {0, 0, 0};

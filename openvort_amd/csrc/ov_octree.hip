@@ -16,10 +16,11 @@
 //     Gamma = sum ds, tensor M_ij = sum ds_i (r - centre)_j (one wavefront per
 //     cell; summation order differs from the reference's by rounding only).
 //  5. Cells are renumbered in preorder; `skip` = subtree size.
-// Traversal (octree.c:275-343), one wavefront per 64 targets and image: the
-// wave walks nodes in preorder; a lane that takes a node (empty, leaf or
-// accepted multipole) waits until the wave leaves that subtree; the wave
-// descends when any active lane opens the node and otherwise jumps over it.
+// Traversal (octree.c:275-343), one lane per (target, image) pair, a few
+// consecutive targets times a group of images per wavefront: the wave walks
+// nodes in preorder; a lane that takes a node (empty, leaf or accepted
+// multipole) waits until the wave leaves that subtree; the wave descends
+// when any active lane opens the node and otherwise jumps over it.
 #include <hip/hip_runtime.h>
 
 #include "ov_device.h"
@@ -532,13 +533,16 @@ struct OctArgs {
   const double* P;   // targets (SoA)
   long cap;
   int n;
-  int n_tgt_tiles;
   const OctNode* nodes;   // n_nodes + 1 (the last one past the end)
   const OctMom* nmom;
   int n_nodes;
   const LeafSeg* segs;
   const ImgDev* img;
   int n_img;
+  int img_per_walk;  // images per wavefront (1..64); 64 / img_per_walk targets
+  int n_groups;      // ceil(n_img / img_per_walk)
+  int n_items;       // (groups of OCT_WAVES wavefronts) x n_groups
+  int per_xcd;       // ceil(n_items / 8)
   OctParams op;
   double kappa4pi;
   double* partial;   // [n_img][3][cap]
@@ -623,80 +627,129 @@ __device__ __forceinline__ bool oct_accept(const double dv[3], double lmax, doub
   return d > 0 && lmax / d < theta;
 }
 
-// Grid: blockIdx.x = group of 4 target tiles, blockIdx.y = image.
-// MIXED: multipoles in FP32 (leaf pairs and all decisions in FP64).
+constexpr int OCT_WAVES = 4;   // wavefronts per block
+
+// Target of image (sh, w2, ax) for the point r (reference shifted(),
+// tangle.c:39-79): r - shift, or the reflection 2 wall - r on axis ax.
+__device__ __forceinline__ void oct_image_target(const double r[3], const double sh[3], double w2, int ax,
+                                                 double rs[3]) {
+  rs[0] = r[0] - sh[0];
+  rs[1] = r[1] - sh[1];
+  rs[2] = r[2] - sh[2];
+  if (__builtin_expect(ax >= 0, 0)) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k == ax) rs[k] = w2 - r[k];
+  }
+}
+
+// rs - (minimum image of c seen from rs), the displacement of octree.c:
+// 286-287 (seg_pwrap moves c by one box length on each periodic axis where
+// |c - rs| > L/2).  Branch-free: fma(k, L, c) with k in {-1, 0, 1} rounds
+// exactly as c - L, c or c + L does.
+__device__ __forceinline__ void oct_min_image(const double rs[3], const double c[3], const BoxDev& b,
+                                              double dv[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double d = c[a] - rs[a];
+    const double h = b.L[a] / 2;
+    const double k = b.periodic[a] ? (d > h ? -1.0 : (d < -h ? 1.0 : 0.0)) : 0.0;
+    dv[a] = rs[a] - fma(k, b.L[a], c[a]);
+  }
+}
+
+// One lane = one (target, image) pair.  A wavefront holds tpw consecutive
+// stream targets x ipw images of one image group (tpw ipw <= 64): the
+// minimum-image opening test (octree.c:286-295) makes a target's images
+// decide alike except near the far faces, and consecutive targets sit next
+// to each other on a filament, so the wave's lanes walk almost the same
+// nodes and most lanes work at every node.  Each lane walks the preorder as
+// octree.c:275-343 does (accept: multipole; leaf: segments; else open): it
+// resumes at node >= next; the wave steps to node + 1 when some lane opened
+// the node and over its subtree otherwise.  Every lane accumulates its own
+// walk's terms in its own order, into the partial plane of its image.
+//
+// Grid: one dimension of n_items = (group of OCT_WAVES waves) x (image
+// group) work items, XCD-major: block b runs on XCD b % 8 (blocks are
+// dispatched round-robin over the 8 XCDs) and takes item (b % 8) per_xcd +
+// b / 8, so each XCD's L2 serves a contiguous range of targets, whose walks
+// touch the same nodes.  MIXED: multipoles in FP32 (leaf pairs and all
+// decisions in FP64).
 template <bool MIXED>
 __device__ __forceinline__ void octree_body(const OctArgs& A) {
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int T = blockIdx.x * 4 + wid;
-  if (T >= A.n_tgt_tiles) return;
-  const int I = blockIdx.y;
-  const ImgDev im = A.img[I];
-  const bool primary = im.primary != 0;
-  const int s = T * TILE + lane;
-  const bool valid = s < A.n;
-  const int st = valid ? s : T * TILE;
-  double r[3] = {A.P[st], A.P[A.cap + st], A.P[2 * A.cap + st]};
-  double rs[3] = {r[0] - im.shift[0], r[1] - im.shift[1], r[2] - im.shift[2]};
-  if (im.axis >= 0) rs[im.axis] = im.wall2 - r[im.axis];
-  const int skip_id = (primary && valid) ? s : -1;
+  const int b = blockIdx.x;
+  const int item = (b & 7) * A.per_xcd + (b >> 3);
+  if (item >= A.n_items) return;
+  const int wgrp = item / A.n_groups;
+  const int g = item - wgrp * A.n_groups;
+  const int W = wgrp * OCT_WAVES + wid;            // target group of this wave
+  const int ipw = A.img_per_walk;
+  const int tpw = TILE / ipw;
+  if (W * tpw >= A.n) return;
+  const int tl = lane / ipw;
+  const int I = g * ipw + (lane - tl * ipw);       // this lane's image
+  const int s = W * tpw + tl;                      // this lane's target
+  const bool valid = tl < tpw && s < A.n && I < A.n_img;
+  const int st = valid ? s : W * tpw;
+  const ImgDev im = A.img[valid ? I : 0];
+  const double r[3] = {A.P[st], A.P[A.cap + st], A.P[2 * A.cap + st]};
+  double rs[3];
+  oct_image_target(r, im.shift, im.wall2, im.axis, rs);
+  const int skip_id = (valid && im.primary) ? s : -1;
+  const bool dipole = A.op.dipole != 0;
   double acc[3] = {0, 0, 0};
-  long leaf_pairs = 0, mps = 0, visits = 0;
-  int next = 0;   // this lane resumes at node >= next
+  unsigned leaf_pairs = 0, mps = 0, visits = 0;
+  int next = valid ? 0 : 0x7fffffff;   // this lane resumes at node >= next
   int node = 0;
-  // The walk's next node is node + 1 (some lane opens this one) or
-  // node + skip: both are loaded before this node is decided, so their
-  // latency overlaps the decision.
-  OctNode nd = A.nodes[0];
   while (node < A.n_nodes) {
-    const OctNode n_in = A.nodes[node + 1];
-    const OctNode n_over = A.nodes[node + nd.skip];
+    const OctNode nd = A.nodes[node];
     ++visits;
+    const int after = node + nd.skip;
     bool open = false;
     if (node >= next) {
       if (nd.n == 0) {
-        next = node + nd.skip;
+        next = after;
       } else if (nd.n == 1 || nd.lmin < A.op.grain) {
         // segs holds one entry past the last sorted point, so the look-ahead
         // load of segment j + 1 stays in bounds
         LeafSeg q = A.segs[nd.first];
         for (int j = nd.first; j < nd.first + nd.n; ++j) {
           const LeafSeg qn = A.segs[j + 1];
+          // the primary sum skips the target's own two segments (octree.c:291)
           if (q.m != skip_id && q.fwd != skip_id) {
             oct_segment(q.a, q.b, rs, acc);
             ++leaf_pairs;
           }
           q = qn;
         }
-        next = node + nd.skip;
+        next = after;
       } else {
-        double cw[3];
-        pwrap(rs, nd.com, A.op.box, cw);
-        const double dv[3] = {rs[0] - cw[0], rs[1] - cw[1], rs[2] - cw[2]};
+        double dv[3];
+        oct_min_image(rs, nd.com, A.op.box, dv);
         if (oct_accept(dv, nd.lmax, A.op.theta)) {
           const OctMom mm = A.nmom[node];
           if (MIXED)
-            oct_multipole32(nd.com, mm, rs, A.op.dipole != 0, acc);
+            oct_multipole32(nd.com, mm, rs, dipole, acc);
           else
-            oct_multipole(nd.com, mm, rs, A.op.dipole != 0, acc);
+            oct_multipole(nd.com, mm, rs, dipole, acc);
           ++mps;
-          next = node + nd.skip;
+          next = after;
         } else {
           open = true;
           next = node + 1;
         }
       }
     }
-    if (__builtin_amdgcn_ballot_w64(open) != 0) {
-      node = node + 1;
-      nd = n_in;
-    } else {
-      node = node + nd.skip;
-      nd = n_over;
-    }
+    node = __builtin_amdgcn_ballot_w64(open) != 0 ? node + 1 : after;
   }
-  if (im.axis >= 0) acc[im.axis] = -acc[im.axis];
+  // mirror image: the velocity is reflected (reference mirror_dir_reflect)
+  if (im.axis >= 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k == im.axis) acc[k] = -acc[k];
+  }
   if (valid) {
     double* out = A.partial + (long)I * 3 * A.cap;
     out[s] = acc[0] * A.kappa4pi;
