@@ -195,6 +195,87 @@ def host_threads():
         return os.cpu_count() or 1
 
 
+def tree_section(ctx, T, tg, common, b):
+    """The octree at theta 0.3 / 0.5 / 0.7: the reference's multipole form
+    (octree.c:303-328) in FP64 and with FP32 multipoles, and the true dipole
+    expansion; per evaluation the device time, the traversal kernel's time,
+    cells, leaf pairs and multipoles, and (when b holds the FP64 direct sum)
+    the error on a 4096-target sample."""
+    rng_t = np.random.default_rng(2002)
+    samp = np.sort(rng_t.choice(tg.active, size=min(4096, len(tg.active)), replace=False))
+    tree = []
+    tree_samples = {}   # theta -> (octree entry, GPU FP64 reference-multipole vels on the sample)
+    for prec, pname, mp in ((T.PREC_FP64, "fp64", T.MP_REFERENCE), (T.PREC_MIXED, "mixed", T.MP_REFERENCE),
+                            (T.PREC_FP64, "fp64", T.MP_DIPOLE)):
+        for th in (0.3, 0.5, 0.7):
+            xt = tg.copy()
+            ctx.reset_stats()
+            ctx.tree_stats()
+            ctx.velocities(xt, 0.0, T.default_params(precision=prec, use_BH=1, BH_resolution=th,
+                                                     multipole=mp, **common), profile=True)
+            ms = ctx.last_device_ms()
+            cells, pairs, mps = ctx.tree_stats()
+            kname = "k_octree_fp64" if prec == T.PREC_FP64 else "k_octree_mixed"
+            tree.append({"theta": th, "precision": pname,
+                         "multipole": "reference" if mp == T.MP_REFERENCE else "dipole",
+                         "eval_ms": round(ms, 2),
+                         "traversal_ms": round(ctx.kernel_stats(kname)[1], 2),
+                         "cells": int(cells), "leaf_pairs": pairs, "multipoles": mps,
+                         "build_kernels_ms": {kn: round(ctx.kernel_stats(kn)[1], 3) for kn in OCT_BUILD}})
+            if b is not None:
+                tree[-1]["err_vels_vs_fp64_direct_sample"] = err_stats(xt.vels[samp], b.vels[samp])
+            if mp == T.MP_REFERENCE and prec == T.PREC_FP64:
+                tree_samples[th] = (len(tree) - 1, xt.vels[samp].copy())
+            log(f"tree {pname} {tree[-1]['multipole']} theta {th}: {ms:.1f} ms, "
+                f"err {tree[-1].get('err_vels_vs_fp64_direct_sample')}")
+    hbm = tree_hbm_bytes()
+    build_hbm = None
+    if hbm:
+        build = tree[0]["build_kernels_ms"]
+        build_hbm = {kn: {"hbm_bytes": hbm[kn], "ms": build[kn],
+                          "GB_per_s": round(hbm[kn] / (build[kn] / 1e3) / 1e9, 1) if build.get(kn) else None}
+                     for kn in OCT_BUILD if kn in hbm}
+    return samp, tree, tree_samples, build_hbm
+
+
+def only_section(args, ctx, T, tg, common):
+    """One optional section on its own (--only), printed as one JSON line."""
+    t0 = time.time()
+    res = {"section": args.only, "points": int(len(tg.active)), "seed": args.seed}
+    if args.only == "octree":
+        res["octree"] = tree_section(ctx, T, tg, common, None)[1]
+    elif args.only == "topology":
+        res["topology"] = topology_section(ctx, T, tg, None, 0)
+    elif args.only == "reconnection":
+        res["reconnection_candidates"] = reconnection_section(ctx, tg)
+    elif args.only == "ensemble":
+        res["ensemble"] = ensemble_section(ctx, T, args)
+    else:
+        res["adjoint"] = adjoint_section(ctx, T)
+    res["wall_s"] = round(time.time() - t0, 1)
+    print(json.dumps(res), flush=True)
+
+
+def reconnection_section(ctx, tg):
+    """Reconnection candidates (the pass-invariant tests of the reference's
+    pair loop) on the workload at rec_dist 1e-3 and 2e-4 cm."""
+    n = len(tg.active)
+    xr = tg.copy()
+    ctx.update_tangents_normals(xr)
+    rec = []
+    for rd in (1e-3, 2e-4):
+        ctx.reconnection_candidates(xr, rd, 0.087)          # sizes the buffers
+        w0 = time.time()
+        cand = ctx.reconnection_candidates(xr, rd, 0.087)
+        wall_ms = (time.time() - w0) * 1e3
+        rec.append({"rec_dist_cm": rd, "rec_angle": 0.087, "candidates": int(len(cand)),
+                    "device_ms": round(ctx.last_device_ms(), 2), "wall_ms": round(wall_ms, 2),
+                    "reference_pair_tests": n * (n - 1) // 2})
+        log(f"reconnection candidates rec_dist {rd}: {len(cand)} pairs, "
+            f"{rec[-1]['device_ms']} ms device, {wall_ms:.1f} ms wall")
+    return rec
+
+
 def main():
     t_start = time.time()
     ap = argparse.ArgumentParser()
@@ -219,6 +300,8 @@ def main():
                     help="Kelvin-ring gradient (4096 points, 200 steps, checkpoints every 20; 0: skip)")
     ap.add_argument("--reconnect", type=int, default=1,
                     help="reconnection candidate search on the workload (0: skip)")
+    ap.add_argument("--only", choices=("octree", "topology", "reconnection", "ensemble", "adjoint"),
+                    help="run this section alone on the same workload (no headline steps), for profiles")
     ap.add_argument("--topology", type=int, default=1,
                     help="device-resident topology edits and the 200-step main loop (0: skip)")
     ap.add_argument("--tree", type=int, default=1,
@@ -262,6 +345,10 @@ def main():
     # workload, not profiled)
     for pp in (p_mx, p_64):
         ctx.rk4_run(T.ring(128), 0.0, 1e-6, 1, pp)
+    if args.only:
+        only_section(args, ctx, T, tg, common)
+        ctx.close()
+        return
 
     out = {}
     # ---- one mixed and one FP64 evaluation of the workload's positions ------
@@ -400,41 +487,9 @@ def main():
     tree_samples = {}   # theta -> (octree entry, GPU FP64 reference-multipole vels on the sample)
     samp = None
     if args.tree > 0 and left() > 60:
-        rng_t = np.random.default_rng(2002)
-        samp = np.sort(rng_t.choice(tg.active, size=min(4096, n), replace=False))
-        tree = []
-        # the reference's multipole form (octree.c:303-328) in FP64 and with FP32
-        # multipoles, and the true dipole expansion for comparison
-        for prec, pname, mp in ((T.PREC_FP64, "fp64", T.MP_REFERENCE), (T.PREC_MIXED, "mixed", T.MP_REFERENCE),
-                                (T.PREC_FP64, "fp64", T.MP_DIPOLE)):
-            for th in (0.3, 0.5, 0.7):
-                xt = tg.copy()
-                ctx.reset_stats()
-                ctx.tree_stats()
-                ctx.velocities(xt, 0.0, T.default_params(precision=prec, use_BH=1, BH_resolution=th,
-                                                         multipole=mp, **common), profile=True)
-                ms = ctx.last_device_ms()
-                cells, pairs, mps = ctx.tree_stats()
-                kname = "k_octree_fp64" if prec == T.PREC_FP64 else "k_octree_mixed"
-                tree.append({"theta": th, "precision": pname,
-                             "multipole": "reference" if mp == T.MP_REFERENCE else "dipole",
-                             "eval_ms": round(ms, 2),
-                             "traversal_ms": round(ctx.kernel_stats(kname)[1], 2),
-                             "cells": int(cells), "leaf_pairs": pairs, "multipoles": mps,
-                             "build_kernels_ms": {kn: round(ctx.kernel_stats(kn)[1], 3) for kn in OCT_BUILD}})
-                if b is not None:
-                    tree[-1]["err_vels_vs_fp64_direct_sample"] = err_stats(xt.vels[samp], b.vels[samp])
-                if mp == T.MP_REFERENCE and prec == T.PREC_FP64:
-                    tree_samples[th] = (len(tree) - 1, xt.vels[samp].copy())
-                log(f"tree {pname} {tree[-1]['multipole']} theta {th}: {ms:.1f} ms, "
-                    f"err {tree[-1].get('err_vels_vs_fp64_direct_sample')}")
-        hbm = tree_hbm_bytes()
-        if hbm:
-            build = tree[0]["build_kernels_ms"]
-            out["octree_build_hbm"] = {
-                kn: {"hbm_bytes": hbm[kn], "ms": build[kn],
-                     "GB_per_s": round(hbm[kn] / (build[kn] / 1e3) / 1e9, 1) if build.get(kn) else None}
-                for kn in OCT_BUILD if kn in hbm}
+        samp, tree, tree_samples, build_hbm = tree_section(ctx, T, tg, common, b)
+        if build_hbm:
+            out["octree_build_hbm"] = build_hbm
         out["octree"] = tree
     elif args.tree > 0:
         skipped.append("octree")
@@ -455,20 +510,7 @@ def main():
         skipped.append("octree_cpu_reference")
 
     if args.reconnect > 0 and left() > 30:
-        xr = tg.copy()
-        ctx.update_tangents_normals(xr)
-        rec = []
-        for rd in (1e-3, 2e-4):
-            ctx.reconnection_candidates(xr, rd, 0.087)          # sizes the buffers
-            w0 = time.time()
-            cand = ctx.reconnection_candidates(xr, rd, 0.087)
-            wall_ms = (time.time() - w0) * 1e3
-            rec.append({"rec_dist_cm": rd, "rec_angle": 0.087, "candidates": int(len(cand)),
-                        "device_ms": round(ctx.last_device_ms(), 2), "wall_ms": round(wall_ms, 2),
-                        "reference_pair_tests": n * (n - 1) // 2})
-            log(f"reconnection candidates rec_dist {rd}: {len(cand)} pairs, "
-                f"{rec[-1]['device_ms']} ms device, {wall_ms:.1f} ms wall")
-        out["reconnection_candidates"] = rec
+        out["reconnection_candidates"] = reconnection_section(ctx, tg)
     elif args.reconnect > 0:
         skipped.append("reconnection_candidates")
 
