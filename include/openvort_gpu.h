@@ -152,7 +152,7 @@ typedef struct {
 enum {
   OV_OK = 0,
   OV_E_INVALID = -1,      /* malformed tangle or arguments */
-  OV_E_UNSUPPORTED = -2,  /* pinned points, walls with pinning, unknown flows */
+  OV_E_UNSUPPORTED = -2,  /* a feature outside an entry point's scope (pinned points off the direct path, ...) */
   OV_E_NONFINITE = -3,    /* a result is NaN/Inf (outputs still written) */
   OV_E_DEVICE = -4,       /* HIP runtime failure */
   OV_E_NOMEM = -5         /* device or host allocation failed */
@@ -179,7 +179,22 @@ const char *ov_last_error(const ov_ctx *ctx);
 /* hipStream_t the context launches on (for callers that order their own work). */
 void *ov_ctx_stream(ov_ctx *ctx);
 
-/* update_tangents_normals (tangle.h:120): writes dxi, tangents, normals. */
+/* update_tangents_normals (tangle.h:120): writes dxi, tangents, normals.
+ *
+ * Walls (this call, ov_gpu_velocities and ov_gpu_rk4_step / ov_gpu_rk4_run
+ * with the direct sum): PINNED and PINNED_SLIP points with pin_wall 0..5
+ * may end open chains (forward or reverse -1).  Neighbours past a chain end
+ * are the mirror images of the chain in that point's wall (step_node,
+ * tangle.c:165-198); PINNED points take the one-sided stencil (tangle.c:
+ * 275-362, forward when the caller's forward index is > 0, as the reference
+ * tests), their non-PINNED neighbours the neighbour-tangent stencil
+ * (tangle.c:364-430); PINNED points get vs = 0 and vels = v_b
+ * (tangle.c:589-594), PINNED_SLIP points lose the velocity normal to their
+ * wall (tangle.c:677-683).  Refused with OV_E_INVALID: a free point without
+ * both links, a pinned point without any, two neighbouring points that both
+ * border a PINNED point (the reference recurses without end there).  The
+ * reference reads status[-1] at a PINNED_SLIP chain end; here that missing
+ * neighbour counts as not PINNED. */
 int ov_gpu_update_tangents_normals(ov_ctx *ctx, ov_tangle_state *tangle);
 
 /* update_velocities (tangle.h:128) at time t: writes vs[] and vels[].

@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) kb_epilogue(const double* __restrict__ P,
   if (s >= ntot) return;
   const int k = tid[s];
   if (k < 0) return;
-  epilogue_point(s, P, T, Nn, lia, bs, !tg[k].lia_only, cap, time, tg[k].ph, vs_out, vel_out, flag);
+  epilogue_point(s, P, T, Nn, lia, bs, !tg[k].lia_only, cap, time, tg[k].ph, -1, vs_out, vel_out, flag);
 }
 
 __global__ void __launch_bounds__(256) kb_wrap(double* __restrict__ X0, long cap,

@@ -86,7 +86,7 @@ struct NearRec {
 struct PhysDev {
   double kappa, a0, alpha, alpha_p, dl_max;
   double max_curv, hyperalpha;
-  ov_flow vs, vn;
+  ov_flow vs, vn, vb;   // vb: velocity of PINNED points (get_vb)
   int lia_only, use_mf, hyperfriction, pad;
 };
 

@@ -25,6 +25,7 @@
 #include "ov_adjoint.hip"
 #include "ov_reconnect.hip"
 #include "ov_topology.hip"
+#include "ov_walls.hip"
 #include "ov_host.hip"
 
 #include <hipcub/hipcub.hpp>
@@ -112,6 +113,11 @@ struct ov_ctx {
   void* t_cub = nullptr;
   size_t t_cub_bytes = 0;
   long* t_stats = nullptr;
+  // points pinned on walls (stream order): pin code per point, see ov_walls.hip
+  bool has_pins = false;
+  std::vector<int> pin_s;
+  int* pin = nullptr;
+  long pin_cap = 0;
   int oct_images_per_walk = 9;   // most images per traversal wavefront (64 / 9 = 7 targets)
   int* t_big = nullptr;
   long t_big_cap = 0;
@@ -376,6 +382,7 @@ int setup_box_physics(ov_ctx* ctx, const ov_domain_box* bx, const ov_boundary_im
   ph.hyperalpha = p->hyperalpha;
   ph.vs = p->vs;
   ph.vn = p->vn;
+  ph.vb = p->vb;
   ctx->flow_time_dep = flow_time_dependent(p->vs.type) ||
                        (p->use_mutual_friction && flow_time_dependent(p->vn.type));
   ph.lia_only = p->LIA_only;
@@ -398,7 +405,7 @@ int setup_box_physics(ov_ctx* ctx, const ov_domain_box* bx, const ov_boundary_im
 // stream order.  EMPTY points must have no forward link (the reference sums
 // every point whose forward link is set, tangle.c:612-615, and every EMPTY
 // point it creates has none, tangle.c:94-99, 1063-1067).
-int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
+int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p, bool allow_pins = false) {
   int rc = 0;
   if (!t || !p) return fail(ctx, OV_E_INVALID, "null tangle or params");
   if (t->N < 0) return fail(ctx, OV_E_INVALID, "negative N");
@@ -412,6 +419,10 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   const int N = t->N;
   std::vector<int> cid(N, -1);
   ctx->act.clear();
+  bool pins = false;
+  auto pinned = [&](int k) {
+    return t->status[k].status == OV_PINNED || t->status[k].status == OV_PINNED_SLIP;
+  };
   for (int k = 0; k < N; ++k) {
     int st = t->status[k].status;
     if (st == OV_EMPTY) {
@@ -419,34 +430,83 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
         return fail(ctx, OV_E_INVALID, "EMPTY point %d has a forward link", k);
       continue;
     }
-    if (st == OV_PINNED || st == OV_PINNED_SLIP)
-      return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are not supported", k);
-    if (st != OV_FREE) return fail(ctx, OV_E_INVALID, "bad status %d at %d", st, k);
-    int f = t->connections[k].forward, r = t->connections[k].reverse;
-    if (f < 0 || f >= N || r < 0 || r >= N)
-      return fail(ctx, OV_E_INVALID, "free point %d has links %d/%d", k, f, r);
-    for (int nb : {f, r}) {
-      int sn = t->status[nb].status;
-      if (sn == OV_PINNED || sn == OV_PINNED_SLIP)
-        return fail(ctx, OV_E_UNSUPPORTED, "point %d neighbours pinned point %d", k, nb);
-      if (sn != OV_FREE) return fail(ctx, OV_E_INVALID, "point %d links to empty point %d", k, nb);
+    if (st == OV_PINNED || st == OV_PINNED_SLIP) {
+      if (!allow_pins)
+        return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are supported by ov_gpu_velocities, "
+                    "ov_gpu_rk4_step/run (direct sum) and ov_gpu_update_tangents_normals only", k);
+      if (t->status[k].pin_wall < 0 || t->status[k].pin_wall > 5)
+        return fail(ctx, OV_E_INVALID, "pinned point %d has pin_wall %d", k, t->status[k].pin_wall);
+      pins = true;
+    } else if (st != OV_FREE) {
+      return fail(ctx, OV_E_INVALID, "bad status %d at %d", st, k);
     }
-    if (t->connections[f].reverse != k)
+    int f = t->connections[k].forward, r = t->connections[k].reverse;
+    // only a pinned point may end an open chain (step_node, tangle.c:176-191)
+    const bool pin_k = st != OV_FREE;
+    if (f >= N || r >= N || (!pin_k && (f < 0 || r < 0)) || f < -1 || r < -1)
+      return fail(ctx, OV_E_INVALID, "%s point %d has links %d/%d", pin_k ? "pinned" : "free", k, f, r);
+    if (f < 0 && r < 0)
+      return fail(ctx, OV_E_INVALID, "pinned point %d has no links (the reference's walk along it "
+                  "does not end)", k);
+    for (int nb : {f, r}) {
+      if (nb < 0) continue;
+      if (t->status[nb].status == OV_EMPTY)
+        return fail(ctx, OV_E_INVALID, "point %d links to empty point %d", k, nb);
+      if (pinned(nb) && !allow_pins)
+        return fail(ctx, OV_E_UNSUPPORTED, "point %d neighbours pinned point %d", k, nb);
+    }
+    if (f >= 0 && t->connections[f].reverse != k)
       return fail(ctx, OV_E_INVALID, "links of %d and %d disagree", k, f);
+    if (r >= 0 && t->connections[r].forward != k)
+      return fail(ctx, OV_E_INVALID, "links of %d and %d disagree", r, k);
     cid[k] = (int)ctx->act.size();
     ctx->act.push_back(k);
   }
+  if (pins) {
+    // a point that is not PINNED but has a PINNED neighbour takes its
+    // tangent from both neighbours' tangents (tangle.c:364-430): it needs
+    // both links, and neither neighbour may need the same rule (the
+    // reference's update_tangent_normal would recurse without end)
+    auto hermite = [&](int k) {
+      if (t->status[k].status == OV_PINNED) return false;
+      int f = t->connections[k].forward, r = t->connections[k].reverse;
+      return (f >= 0 && t->status[f].status == OV_PINNED) || (r >= 0 && t->status[r].status == OV_PINNED);
+    };
+    for (int k : ctx->act) {
+      if (!hermite(k)) continue;
+      int f = t->connections[k].forward, r = t->connections[k].reverse;
+      if (f < 0 || r < 0)
+        return fail(ctx, OV_E_UNSUPPORTED, "PINNED_SLIP chain end %d next to a PINNED point (the "
+                    "reference reads the tangent of link -1)", k);
+      if (hermite(f) || hermite(r))
+        return fail(ctx, OV_E_INVALID, "points %d and %d between PINNED points: the reference's "
+                    "update_tangent_normal recurses without end on two-point chains", k,
+                    hermite(f) ? f : r);
+    }
+  }
   const int n = (int)ctx->act.size();
-  // stream order: loops in order of their lowest caller index, each walked forward
+  // stream order: loops (and open chains) in order of their lowest caller
+  // index; a loop walked forward from that index, a chain from its head
   std::vector<int> order;
   order.reserve(n);
   std::vector<char> seen(n, 0);
   for (int i = 0; i < n; ++i) {
+    if (seen[i]) continue;
     int j = i;
-    while (!seen[j]) {
+    if (pins) {
+      // back to the head of an open chain (or once around a loop)
+      int steps = 0;
+      for (int r = t->connections[ctx->act[j]].reverse; r >= 0 && cid[r] != i && steps <= n; ++steps) {
+        j = cid[r];
+        r = t->connections[ctx->act[j]].reverse;
+      }
+      if (t->connections[ctx->act[j]].reverse >= 0) j = i;   // closed loop
+    }
+    while (j >= 0 && !seen[j]) {
       seen[j] = 1;
       order.push_back(j);
-      j = cid[t->connections[ctx->act[j]].forward];
+      const int f = t->connections[ctx->act[j]].forward;
+      j = f >= 0 ? cid[f] : -1;
     }
   }
   std::vector<int> pos(n);
@@ -455,11 +515,17 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   std::vector<int> act_s(n);
   ctx->fwd_s.assign(n, 0);
   ctx->rev_s.assign(n, 0);
+  ctx->has_pins = pins;
+  ctx->pin_s.assign(pins ? n : 0, -1);
   for (int q = 0; q < n; ++q) {
     int k = ctx->act[order[q]];
     act_s[q] = k;
-    ctx->fwd_s[q] = pos[cid[t->connections[k].forward]];
-    ctx->rev_s[q] = pos[cid[t->connections[k].reverse]];
+    const int f = t->connections[k].forward, r = t->connections[k].reverse;
+    ctx->fwd_s[q] = f >= 0 ? pos[cid[f]] : -1;
+    ctx->rev_s[q] = r >= 0 ? pos[cid[r]] : -1;
+    if (pins && pinned(k))
+      ctx->pin_s[q] = t->status[k].pin_wall | (t->status[k].status == OV_PINNED_SLIP ? 8 : 0) |
+                      (f > 0 ? 16 : 0);
   }
   ctx->act.swap(act_s);
   ctx->n = n;
@@ -556,6 +622,14 @@ int download_scalar_at(ov_ctx* ctx, const double* src, long o, double* dst) {
 
 int upload_topology(ov_ctx* ctx) {
   const int n = ctx->n;
+  if (ctx->has_pins) {
+    if (ctx->pin_cap < ctx->cap) {
+      int rc = dalloc(ctx, (void**)&ctx->pin, sizeof(int) * ctx->cap);
+      if (rc) return rc;
+      ctx->pin_cap = ctx->cap;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->pin, ctx->pin_s.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  }
   HIPCHK(hipMemcpyAsync(ctx->fwd, ctx->fwd_s.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(ctx->rev, ctx->rev_s.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(ctx->img, ctx->imgs.data(), sizeof(ImgDev) * ctx->imgs.size(),
@@ -831,16 +905,36 @@ int tree_sum(ov_ctx* ctx, const double* X) {
 
 // One velocity evaluation at positions X: geometry (or the caller's s', s''
 // already in Tg/Ng), the direct sum, the epilogue.
+// dxi, s', s'' (when geom) and the local induction at positions X: one pass
+// for free points, three when points are pinned on walls (dxi, then every
+// point but those next to a PINNED one, then those).
+int geometry(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, double* dxi) {
+  const int n = ctx->n;
+  const long cap = ctx->cap;
+  int rc;
+  if (!ctx->has_pins)
+    return launch(ctx, "k_geometry", k_geometry, grid1(n), dim3(256), X, (const int*)ctx->fwd,
+                  (const int*)ctx->rev, n, cap, ctx->box, ctx->ph, geom ? 1 : 0, Tg, Ng, dxi, ctx->lia);
+  if (geom && (rc = launch(ctx, "k_wall_dxi", k_wall_dxi, grid1(n), dim3(256), X, (const int*)ctx->fwd,
+                           (const int*)ctx->rev, (const int*)ctx->pin, n, cap, ctx->box, dxi, ctx->flag)))
+    return rc;
+  for (int phase = 1; phase <= (geom ? 2 : 1); ++phase)
+    if ((rc = launch(ctx, "k_wall_geometry", k_wall_geometry, grid1(n), dim3(256), X, (const int*)ctx->fwd,
+                     (const int*)ctx->rev, (const int*)ctx->pin, n, cap, ctx->box, ctx->ph, geom ? 1 : 0,
+                     phase, Tg, Ng, (const double*)dxi, ctx->lia, ctx->flag)))
+      return rc;
+  return OV_OK;
+}
+
 int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, double* dxi,
              double* vs, double* vel, double time) {
   const int n = ctx->n;
   const long cap = ctx->cap;
   int rc;
   if (n == 0) return OV_OK;
-  if ((rc = launch(ctx, "k_geometry", k_geometry, grid1(n), dim3(256), X, (const int*)ctx->fwd,
-                   (const int*)ctx->rev, n, cap, ctx->box, ctx->ph, geom ? 1 : 0, Tg, Ng, dxi,
-                   ctx->lia)))
-    return rc;
+  if (ctx->has_pins && ctx->use_tree && !ctx->lia_only)
+    return fail(ctx, OV_E_UNSUPPORTED, "points pinned on walls: direct sum only (use_BH = 0)");
+  if ((rc = geometry(ctx, X, geom, Tg, Ng, dxi))) return rc;
   int have_bs = 0;
   if (!ctx->lia_only && ctx->use_tree) {
     if ((rc = tree_sum(ctx, X))) return rc;
@@ -865,6 +959,10 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
                      (const int*)ctx->counts, ctx->src, ctx->far)))
       return rc;
     if ((rc = launch(ctx, "k_tgt_tiles", k_tgt_tiles, dim3(n_tgt), dim3(TILE), X, n, cap, ctx->tgt)))
+      return rc;
+    // an open chain's end has no segment: its adjacency entry stays -1
+    if (ctx->has_pins &&
+        (rc = launch(ctx, "k_fill_int", k_fill_int, grid1(2 * n), dim3(256), ctx->adj, 2 * n, -1)))
       return rc;
     if ((rc = launch(ctx, "k_adjacent", k_adjacent, dim3((unsigned)((2 * n + 255) / 256)), dim3(256),
                      (const NearRec*)ctx->near, (const int*)ctx->counts, ctx->adj)))
@@ -913,7 +1011,8 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
   }
   if ((rc = launch(ctx, "k_epilogue", k_epilogue, grid1(n), dim3(256), X, (const double*)Tg,
                    (const double*)Ng, (const double*)ctx->lia, (const double*)ctx->bs, have_bs, n,
-                   cap, time, ctx->ph, vs, vel, ctx->flag)))
+                   cap, time, ctx->ph, ctx->has_pins ? (const int*)ctx->pin : (const int*)nullptr, vs, vel,
+                   ctx->flag)))
     return rc;
   return OV_OK;
 }
@@ -1684,7 +1783,7 @@ void ov_ctx_destroy(ov_ctx* ctx) {
   for (double** b : v)
     if (*b) hipFree(*b);
   void* o[] = {ctx->fwd,  ctx->rev, ctx->flag, ctx->brk, ctx->rank, ctx->bsum, ctx->counts, ctx->adj,
-               ctx->tgt,  ctx->src, ctx->far,  ctx->near, ctx->img};
+               ctx->tgt,  ctx->src, ctx->far,  ctx->near, ctx->img, ctx->pin};
   for (void* b : o)
     if (b) hipFree(b);
   void* tb[] = {ctx->t_order, ctx->t_k[0], ctx->t_k[1], ctx->t_k[2], ctx->t_ks[0], ctx->t_ks[1],
@@ -1764,15 +1863,12 @@ int ov_gpu_update_tangents_normals(ov_ctx* ctx, ov_tangle_state* t) {
   ov_params_default(&p);
   p.LIA_only = 1;
   HIPCHK(hipSetDevice(ctx->device));
-  int rc = prepare(ctx, t, &p);
+  int rc = prepare(ctx, t, &p, true);
   if (rc) return rc;
   if (ctx->n == 0) return OV_OK;
   if ((rc = upload_topology(ctx)) || (rc = upload_vec3(ctx, t->vnodes, ctx->X0))) return rc;
   if ((rc = mark_begin(ctx))) return rc;
-  if ((rc = launch(ctx, "k_geometry", k_geometry, grid1(ctx->n), dim3(256), (const double*)ctx->X0,
-                   (const int*)ctx->fwd, (const int*)ctx->rev, ctx->n, ctx->cap, ctx->box, ctx->ph, 1,
-                   ctx->T, ctx->Nn, ctx->dxi, ctx->lia)))
-    return rc;
+  if ((rc = geometry(ctx, ctx->X0, true, ctx->T, ctx->Nn, ctx->dxi))) return rc;
   int flag = 0;
   if ((rc = finish(ctx, &flag))) return rc;
   if ((rc = download_vec3(ctx, ctx->T, t->tangents)) || (rc = download_vec3(ctx, ctx->Nn, t->normals)) ||
@@ -1784,7 +1880,7 @@ int ov_gpu_update_tangents_normals(ov_ctx* ctx, ov_tangle_state* t) {
 int ov_gpu_velocities(ov_ctx* ctx, ov_tangle_state* t, double time, const ov_params* p, int flags) {
   if (!ctx) return OV_E_INVALID;
   HIPCHK(hipSetDevice(ctx->device));
-  int rc = prepare(ctx, t, p);
+  int rc = prepare(ctx, t, p, true);
   if (rc) return rc;
   if (ctx->n == 0) return OV_OK;
   ctx->profile = (flags & OV_PROFILE) != 0;
@@ -1815,7 +1911,7 @@ static int rk4_common(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int
   if (!ctx) return OV_E_INVALID;
   if (nsteps < 0) return fail(ctx, OV_E_INVALID, "negative step count");
   HIPCHK(hipSetDevice(ctx->device));
-  int rc = prepare(ctx, t, p);
+  int rc = prepare(ctx, t, p, true);
   if (rc) return rc;
   if (ctx->n == 0 || nsteps == 0) return OV_OK;
   ctx->profile = (flags & OV_PROFILE) != 0;

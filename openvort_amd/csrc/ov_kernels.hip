@@ -197,11 +197,13 @@ __device__ __forceinline__ int block_exscan256(int v, int* lds, int* total) {
   return base + x - v;
 }
 
-// Segment s -> fwd[s] ends exactly at stream point s+1 ("continues") unless
-// it closes a loop or its end is the periodic image of the forward point
-// (reference seg_pwrap, vec3_maths.c:105-138).  Points whose segment does
-// not continue emit a second record.  Per block: exclusive scan of those
-// flags (rank[s]) and the block total.
+// Segment s -> fwd[s] ends exactly at stream point s+1 ("continues",
+// brk = 0) unless it closes a loop or its end is the periodic image of the
+// forward point (reference seg_pwrap, vec3_maths.c:105-138): brk = 1, and
+// the point emits a second record.  A point without a forward link (the
+// pinned end of an open chain) has no segment: brk = 2, no second record.
+// Per block: exclusive scan of the second-record flags (rank[s]) and the
+// block total.
 __global__ void __launch_bounds__(256) k_cont(const double* __restrict__ P,
                                               const int* __restrict__ fwd, int n, long cap,
                                               BoxDev box, int* __restrict__ brk,
@@ -211,13 +213,17 @@ __global__ void __launch_bounds__(256) k_cont(const double* __restrict__ P,
   int nc = 0;
   if (s < n) {
     int f = fwd[s];
-    double a[3], pf[3], b[3];
-    load3(P, cap, s, a);
-    load3(P, cap, f, pf);
-    pwrap(a, pf, box, b);
-    bool cont = (f == s + 1) && b[0] == pf[0] && b[1] == pf[1] && b[2] == pf[2];
-    nc = cont ? 0 : 1;
-    brk[s] = nc;
+    if (f < 0) {
+      brk[s] = 2;
+    } else {
+      double a[3], pf[3], b[3];
+      load3(P, cap, s, a);
+      load3(P, cap, f, pf);
+      pwrap(a, pf, box, b);
+      bool cont = (f == s + 1) && b[0] == pf[0] && b[1] == pf[1] && b[2] == pf[2];
+      nc = cont ? 0 : 1;
+      brk[s] = nc;
+    }
   }
   int tot;
   int r = block_exscan256(nc, lds, &tot);
@@ -290,7 +296,7 @@ __global__ void __launch_bounds__(256) k_records(const double* __restrict__ P,
     r0.seg_fwd = -1;
   }
   near[off] = r0;
-  if (brk[s]) {
+  if (brk[s] == 1) {
     int f = fwd[s];
     double pf[3], b[3];
     load3(P, cap, f, pf);
@@ -974,11 +980,20 @@ __device__ __forceinline__ void epilogue_point(int s, const double* __restrict__
                                                const double* __restrict__ lia,
                                                const double* __restrict__ bs, int have_bs,
                                                long cap, double time, const PhysDev& ph,
-                                               double* __restrict__ vs_out,
+                                               int pc, double* __restrict__ vs_out,
                                                double* __restrict__ vel_out,
                                                int* __restrict__ flag) {
   double r[3], t[3], nm[3], l[3], vs[3], v[3], ext[3];
   load3(P, cap, s, r);
+  if (pc >= 0 && !((pc >> 3) & 1)) {
+    // PINNED: v_s = 0, the point moves with the boundary (tangle.c:589-594)
+    const double z[3] = {0, 0, 0};
+    flow_eval(ph.vb, r, time, v);
+    store3(vs_out, cap, s, z);
+    store3(vel_out, cap, s, v);
+    if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) atomicOr(flag, 1);
+    return;
+  }
   load3(T, cap, s, t);
   load3(Nn, cap, s, nm);
   load3(lia, cap, s, l);
@@ -1015,6 +1030,18 @@ __device__ __forceinline__ void epilogue_point(int s, const double* __restrict__
       v[i] += a2[i] * -ph.alpha_p;
     }
   }
+  if (pc >= 0) {
+    // PINNED_SLIP: the velocity normal to the wall is removed (tangle.c:677-683)
+    const int ax = (pc & 7) >> 1;
+    const double sg = (pc & 1) ? -1.0 : 1.0;   // boundary_normals: +e on the low face
+    double nrm[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a == ax) nrm[a] = sg;
+    const double nv = nrm[0] * v[0] + nrm[1] * v[1] + nrm[2] * v[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] -= nrm[i] * nv;
+  }
   store3(vs_out, cap, s, vs);
   store3(vel_out, cap, s, v);
   bool bad = !(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]) && isfinite(vs[0]) &&
@@ -1028,12 +1055,13 @@ __global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ P,
                                                   const double* __restrict__ lia,
                                                   const double* __restrict__ bs, int have_bs, int n,
                                                   long cap, double time, PhysDev ph,
+                                                  const int* __restrict__ pin,
                                                   double* __restrict__ vs_out,
                                                   double* __restrict__ vel_out,
                                                   int* __restrict__ flag) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  epilogue_point(s, P, T, Nn, lia, bs, have_bs, cap, time, ph, vs_out, vel_out, flag);
+  epilogue_point(s, P, T, Nn, lia, bs, have_bs, cap, time, ph, pin ? pin[s] : -1, vs_out, vel_out, flag);
 }
 
 // ---------------------------------------------------------------------------

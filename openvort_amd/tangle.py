@@ -374,18 +374,28 @@ class Tangle:
 
 
 class TangleBuilder:
-    """Accumulates closed filaments and produces a Tangle."""
+    """Accumulates closed filaments and open chains pinned on walls, and
+    produces a Tangle."""
 
     def __init__(self):
         self.pts = []
         self.n = 0
         self.loops = []
+        self.chains = {}      # start -> ((head status, wall), (tail status, wall))
 
     def add_loop(self, pts: np.ndarray) -> None:
         pts = np.asarray(pts, dtype=np.float64).reshape(-1, 3)
         self.pts.append(pts)
         self.loops.append((self.n, len(pts)))
         self.n += len(pts)
+
+    def add_chain(self, pts: np.ndarray, head=(PINNED, Z_L), tail=(PINNED, Z_H)) -> None:
+        """An open filament from pts[0] to pts[-1]: the head has no reverse
+        link, the tail no forward link, both are pinned, (status, pin_wall)
+        each (a vortex line ending on walls, reference tangle.h:36-44)."""
+        start = self.n
+        self.add_loop(pts)
+        self.chains[start] = (head, tail)
 
     def add_ring(self, center, direction, r: float, npts: int) -> None:
         """Ring geometry of the reference add_circle (src/vortex_utils.c:29-99):
@@ -434,6 +444,12 @@ class TangleBuilder:
             con[idx, 0] = np.roll(idx, -1)
             con[idx, 1] = np.roll(idx, 1)
             st[idx, 0] = FREE
+            if start in self.chains:
+                (hs, hw), (ts_, tw) = self.chains[start]
+                con[idx[0], 1] = -1
+                con[idx[-1], 0] = -1
+                st[idx[0]] = (hs, hw)
+                st[idx[-1]] = (ts_, tw)
         return Tangle(vn, con, st, tuple(box_lo), tuple(box_hi), tuple(walls), list(images))
 
 

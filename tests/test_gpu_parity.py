@@ -162,10 +162,20 @@ def test_errors(ctx):
     t = T.ring(32)
     p = ring_params()
     bad = t.copy()
-    bad.status[3, 0] = T.PINNED
+    bad.status[3, 0] = T.PINNED          # pinned, but on no wall (pin_wall -1)
     with pytest.raises(OVError) as e:
         ctx.velocities(bad, 0.0, p)
+    assert e.value.code == OV_E_INVALID
+    bad = t.copy()
+    bad.status[3] = (T.PINNED, T.Z_L)    # walls: direct sum only
+    with pytest.raises(OVError) as e:
+        ctx.velocities(bad, 0.0, ring_params(use_BH=1))
     assert e.value.code == OV_E_UNSUPPORTED
+    bad = t.copy()
+    bad.connections[5, 0] = -1           # a free point must have both links
+    with pytest.raises(OVError) as e:
+        ctx.velocities(bad, 0.0, p)
+    assert e.value.code == OV_E_INVALID
     bad = t.copy()
     bad.status[3, 0] = T.EMPTY
     with pytest.raises(OVError) as e:

@@ -1,0 +1,231 @@
+"""Vortex lines pinned on walls against the reference's own functions
+(src/tangle.c compiled into oracle/_ref): step_node / arc_length with the
+mirror at chain ends, the one-sided stencil of PINNED points and the
+neighbour-tangent stencil next to them (tangle.c:165-480), the PINNED
+boundary velocity and PINNED_SLIP projection (tangle.c:589-594, 677-683),
+and RK4 steps of such tangles.
+
+Box 0.1 cm, periodic in x and y, mirror walls at z = 0 and z = 0.1 with
+their mirror images; lines run from wall to wall with Kelvin-wave
+displacements, next to closed rings.  Tolerances: dxi 1e-14 and tangents
+1e-12 relative; normals 1e-11 of their RMS and 1e-9 RMS relative (the
+stencil coefficients are formed in a different but equivalent order, and
+s'' of a nearly straight line is small); velocities 1e-11 relative (FP64)
+and 1e-6 (mixed); positions 1e-15 cm per step.
+
+The reference reads status[-1] at a PINNED_SLIP chain end (tangle.c:365-366,
+`next` or `prev` is -1 there); the tests give the status array a leading
+FREE entry so that this read is defined, and the library treats the missing
+neighbour as not PINNED, which is what that entry makes the reference do.
+"""
+import numpy as np
+import pytest
+
+from openvort_amd import tangle as T
+from openvort_amd.gpu import OV_E_INVALID, OV_E_UNSUPPORTED, OVError
+from oracle.oracle import Reference
+from tests.helpers import rel_err
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not Reference.available(), reason="reference library not built")]
+
+L = 0.1
+WALLS = (T.WALL_PERIODIC,) * 4 + (T.WALL_MIRROR,) * 2
+IMAGES = [(sx, sy, 0, -1) for sx, sy in ((1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (1, -1), (-1, 1), (-1, -1))] + \
+    [(0, 0, 0, T.Z_L), (0, 0, 0, T.Z_H)]
+
+
+def padded(t):
+    """t with its status array one entry into a buffer whose first entry is
+    FREE (the reference's status[-1] read at PINNED_SLIP ends)."""
+    st = np.zeros((t.N + 1, 2), dtype=np.int32)
+    st[0] = (T.FREE, -1)
+    st[1:] = t.status
+    t.status = st[1:]
+    return t
+
+
+def copy(t):
+    return padded(t.copy())
+
+
+def wall_line(rng, npts, x0, y0, amp=2e-3):
+    z = np.linspace(0.0, L, npts)
+    z[1:-1] += rng.uniform(-0.2, 0.2, npts - 2) * (L / (npts - 1))
+    x = np.full(npts, x0)
+    y = np.full(npts, y0)
+    for m in (1, 2, 3):
+        a, b = rng.uniform(-amp, amp, 2) / m
+        x += a * np.sin(np.pi * m * z / L)
+        y += b * np.sin(np.pi * m * z / L)
+    return np.stack([x, y, z], 1)
+
+
+def wall_tangle(seed=3, status=T.PINNED, n_lines=3, npts=48, rings=True, extra_chains=(), holes=0):
+    rng = np.random.default_rng(seed)
+    b = T.TangleBuilder()
+    for q in range(n_lines):
+        b.add_chain(wall_line(rng, npts, *rng.uniform(0.02, 0.08, 2)), (status, T.Z_L), (status, T.Z_H))
+    if rings:
+        b.add_ring((0.05, 0.05, 0.05), (0.3, 0.2, 0.93), 0.012, 60)
+        b.add_ring((0.03, 0.07, 0.04), (0.0, 1.0, 0.0), 0.008, 40)
+    for pts in extra_chains:
+        b.add_chain(pts, (status, T.Z_L), (status, T.Z_H))
+    t = b.build((0, 0, 0), (L, L, L), WALLS, IMAGES, holes=holes, seed=seed)
+    return padded(t)
+
+
+def params(**kw):
+    base = dict(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016)
+    base.update(kw)
+    return T.default_params(**base)
+
+
+def _geometry(a, b):
+    act = a.active
+    np.testing.assert_allclose(b.dxi[act], a.dxi[act], rtol=1e-14, atol=0)
+    et, en = rel_err(b.tangents[act], a.tangents[act]), rel_err(b.normals[act], a.normals[act])
+    print("tangents", et, "normals", en, flush=True)
+    # s'' of a nearly straight line is tiny at some points, where the
+    # stencil's rounding (weights ~ 1/dxi^2) dominates its relative error:
+    # normals are bounded against their RMS and in RMS relative error
+    assert et[0] < 1e-12 and en[2] < 1e-11 and en[1] < 1e-9
+
+
+@pytest.mark.parametrize("status", [T.PINNED, T.PINNED_SLIP])
+def test_wall_geometry_matches_reference(ctx, status):
+    t = wall_tangle(status=status, holes=17)
+    a, b = copy(t), copy(t)
+    Reference().update_tangents_normals(a)
+    ctx.update_tangents_normals(b)
+    _geometry(a, b)
+
+
+@pytest.mark.parametrize("status", [T.PINNED, T.PINNED_SLIP])
+@pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
+def test_wall_velocities_match_reference(ctx, status, prec):
+    t = wall_tangle(status=status, holes=17)
+    p64 = params()
+    a, b = copy(t), copy(t)
+    R = Reference()
+    R.set_params(p64)
+    R.update_tangle(a, 0.0, p64)
+    ctx.velocities(b, 0.0, params(precision=prec, near_radius=0.01))
+    act = a.active
+    _geometry(a, b)
+    e_vs, e_v = rel_err(b.vs[act], a.vs[act]), rel_err(b.vels[act], a.vels[act])
+    print("vs", e_vs, "vels", e_v, flush=True)
+    tol = 1e-11 if prec == T.PREC_FP64 else 1e-6
+    ends = act[a.status[act, 0] != T.FREE]
+    free = act[a.status[act, 0] == T.FREE]
+    assert rel_err(b.vels[free], a.vels[free])[0] < tol and rel_err(b.vs[free], a.vs[free])[0] < tol
+    if status == T.PINNED:
+        # v_s = 0 and the wall's velocity (no flow) at pinned points
+        assert np.all(b.vs[ends] == 0) and np.all(b.vels[ends] == 0)
+        assert np.all(a.vels[ends] == 0)
+    else:
+        # no velocity normal to the wall (z) at slipping ends
+        assert np.all(b.vels[ends, 2] == 0) and np.all(a.vels[ends, 2] == 0)
+        assert rel_err(b.vels[ends], a.vels[ends])[0] < tol
+
+
+def test_wall_boundary_velocity(ctx):
+    """PINNED points move with the boundary's imposed flow v_b."""
+    t = wall_tangle()
+    p = params(vb=(0.1, -0.2, 0.0))
+    a, b = copy(t), copy(t)
+    R = Reference()
+    R.set_params(p)
+    R.update_tangle(a, 0.0, p)
+    ctx.velocities(b, 0.0, p)
+    act = a.active
+    ends = act[a.status[act, 0] == T.PINNED]
+    np.testing.assert_array_equal(b.vels[ends], np.tile([0.1, -0.2, 0.0], (len(ends), 1)))
+    np.testing.assert_array_equal(a.vels[ends], b.vels[ends])
+    assert rel_err(b.vels[act], a.vels[act])[0] < 1e-11
+
+
+def _permuted(t, first):
+    """t with its points renumbered so that point `first` gets index 0."""
+    N = t.N
+    perm = np.arange(N)
+    perm[[0, first]] = perm[[first, 0]]          # old index -> new index
+    inv = np.argsort(perm)
+    con = t.connections[inv].copy()
+    con = np.where(con >= 0, perm[np.maximum(con, 0)], -1).astype(np.int32)
+    u = T.Tangle(t.vnodes[inv].copy(), con, t.status[inv].copy(), t.box_lo, t.box_hi, t.walls,
+                 list(t.images))
+    return padded(u)
+
+
+def test_wall_forward_index_zero(ctx):
+    """A PINNED head whose forward point has index 0 takes the backward
+    one-sided stencil (the reference tests forward > 0, tangle.c:285)."""
+    t = wall_tangle(n_lines=2, rings=False)
+    head = int(np.nonzero((t.status[:, 0] == T.PINNED) & (t.connections[:, 1] == -1))[0][0])
+    u = _permuted(t, int(t.connections[head, 0]))
+    h2 = int(np.nonzero((u.status[:, 0] == T.PINNED) & (u.connections[:, 1] == -1))[0][0])
+    assert u.connections[h2, 0] == 0
+    a, b = copy(u), copy(u)
+    Reference().update_tangents_normals(a)
+    ctx.update_tangents_normals(b)
+    _geometry(a, b)
+    # the same tangle without the renumbering differs at that head
+    c = copy(t)
+    ctx.update_tangents_normals(c)
+    assert np.abs(c.normals[head] - b.normals[h2]).max() > 0
+
+
+def test_wall_short_chains(ctx):
+    """Chains of 3 and 5 points (one and three free points between the
+    pins) match the reference; a 4-point chain is refused (the reference's
+    update_tangent_normal recurses without end on it)."""
+    rng = np.random.default_rng(9)
+    short = [wall_line(rng, 3, 0.02, 0.03), wall_line(rng, 5, 0.07, 0.06)]
+    t = wall_tangle(n_lines=1, extra_chains=short)
+    p = params()
+    a, b = copy(t), copy(t)
+    R = Reference()
+    R.set_params(p)
+    R.update_tangle(a, 0.0, p)
+    ctx.velocities(b, 0.0, p)
+    _geometry(a, b)
+    act = a.active
+    assert rel_err(b.vels[act], a.vels[act])[0] < 1e-11
+    bad = wall_tangle(n_lines=1, extra_chains=[wall_line(rng, 4, 0.05, 0.05)])
+    with pytest.raises(OVError) as e:
+        ctx.velocities(copy(bad), 0.0, p)
+    assert e.value.code == OV_E_INVALID
+
+
+def test_wall_rk4_matches_reference(ctx):
+    """Three steps of update_tangle + rk4_step + enforce_boundaries with
+    PINNED and PINNED_SLIP lines, against the reference's own functions."""
+    for status in (T.PINNED, T.PINNED_SLIP):
+        t = wall_tangle(seed=11, status=status)
+        p = params()
+        dt = 1e-5
+        a, b = copy(t), copy(t)
+        R = Reference()
+        R.set_params(p)
+        for k in range(3):
+            R.step(a, k * dt, dt, p)
+            R.enforce_boundaries(a)
+        ctx.rk4_run(b, 0.0, dt, 3, p, enforce_boundaries=True)
+        act = a.active
+        d = np.abs(a.vnodes[act] - b.vnodes[act]).max()
+        moved = np.abs(a.vnodes[act] - t.vnodes[act]).max()
+        print(f"status {status}: max position diff {d:.3g} (moved {moved:.3g})", flush=True)
+        assert d < 1e-15 and moved > 1e-9
+        ends = act[a.status[act, 0] != T.FREE]
+        # ends stay on their walls
+        np.testing.assert_array_equal(b.vnodes[ends, 2], t.vnodes[ends, 2])
+
+
+def test_wall_unsupported_paths(ctx):
+    t = wall_tangle()
+    for fn in (lambda: ctx.velocities(copy(t), 0.0, params(use_BH=1)),
+               lambda: ctx.rk4_run_batch([copy(t)], 0.0, 1e-5, 1, params())):
+        with pytest.raises(OVError) as e:
+            fn()
+        assert e.value.code == OV_E_UNSUPPORTED
