@@ -341,7 +341,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A) {
   }
 }
 
-__global__ void __launch_bounds__(256) kb_direct_mixed(BatchArgs A) { batch_body<true, true>(A); }
+__global__ void __launch_bounds__(256, 4) kb_direct_mixed(BatchArgs A) { batch_body<true, true>(A); }
 __global__ void __launch_bounds__(256) kb_direct_fp64(BatchArgs A) { batch_body<false, true>(A); }
 __global__ void __launch_bounds__(256, 4) kb_grid_mixed(BatchArgs A) { batch_body<true, false>(A); }
 __global__ void __launch_bounds__(256) kb_grid_fp64(BatchArgs A) { batch_body<false, false>(A); }
