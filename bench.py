@@ -238,6 +238,81 @@ def tree_section(ctx, T, tg, common, b):
     return samp, tree, tree_samples, build_hbm
 
 
+def configs_section(ctx, T, ref, threads, full):
+    """The per-configuration CPU plan: each configuration's GPU figure beside
+    the reference's own C (oracle/_ref, `threads` OpenMP threads).
+    256-point ring, open, FP64, dt 1e-5 s, 100 steps, hyperfriction
+    off; GPU and CPU positions compared.  configs[1]: 64 rings (seed 1001,
+    periodic-26, v_n = 0.5 cm/s along x): one FP64 and one mixed evaluation on
+    the GPU, mixed vs FP64 over every point, 200 mixed RK4 steps at dt 5e-6 s;
+    the CPU evaluation of every point (full) or of 1024 seeded targets,
+    scaled to the whole evaluation.  configs[4]: the 4096-point Kelvin ring,
+    FP64 reference RK4 at dt 1e-7 s: 200 GPU steps, 3 CPU steps scaled to 200."""
+    res = {}
+    p0 = T.default_params(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, hyperfriction=0)
+    g = T.ring(256)
+    ctx.rk4_run(g, 0.0, 1e-5, 100, p0)
+    c0 = {"points": 256, "steps": 100, "gpu_device_ms": round(ctx.last_device_ms(), 2)}
+    if ref is not None:
+        a = T.ring(256)
+        ref.set_params(p0)
+        w0 = time.time()
+        for k in range(100):
+            ref.step(a, k * 1e-5, 1e-5, p0)
+        c0["cpu_reference_ms"] = round((time.time() - w0) * 1e3, 1)
+        c0["max_position_diff_cm"] = float(np.abs(a.vnodes - g.vnodes).max())
+    res["cfg0_ring256"] = c0
+
+    t1 = T.random_rings(1001, 0.1, n_rings=64)
+    n1 = len(t1.active)
+    common = dict(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, vn=(0.5, 0.0, 0.0))
+    p64 = T.default_params(**common)
+    pmx = T.default_params(precision=T.PREC_MIXED, near_radius=0.01, **common)
+    a, b = t1.copy(), t1.copy()
+    ctx.velocities(a, 0.0, p64)
+    f64 = ctx.last_device_ms()
+    inter = ctx.last_interactions()
+    ctx.velocities(b, 0.0, pmx)
+    mx = ctx.last_device_ms()
+    act = t1.active
+    x = t1.copy()
+    ctx.rk4_run(x, 0.0, 5e-6, 200, pmx, enforce_boundaries=True)
+    c1 = {"points": n1, "interactions_per_eval": inter, "gpu_fp64_eval_ms": round(f64, 3),
+          "gpu_mixed_eval_ms": round(mx, 3), "mixed_vs_fp64": round(f64 / mx, 3),
+          "err_vels_mixed_vs_fp64": err_stats(b.vels[act], a.vels[act]),
+          "gpu_mixed_rk4_steps": 200, "gpu_mixed_ms_per_step": round(ctx.last_device_ms() / 200, 3)}
+    if ref is not None:
+        c = t1.copy()
+        ref.update_tangents_normals(c)
+        idx = act if full else np.sort(np.random.default_rng(2002).choice(act, 1024, replace=False))
+        idx = idx.astype(np.int32)
+        w0 = time.time()
+        _, vel = ref.velocities_sample(c, 0.0, p64, idx)
+        cpu_s = time.time() - w0
+        c1["cpu_targets"] = int(len(idx))
+        c1["cpu_threads"] = threads
+        c1["cpu_reference_eval_s"] = round(cpu_s * n1 / len(idx), 2)
+        c1["cpu_interactions_per_s"] = inter * len(idx) / n1 / cpu_s
+        c1["err_vels_gpu_fp64_vs_reference"] = err_stats(a.vels[idx], vel)
+    res["cfg1_64_rings"] = c1
+
+    k4 = T.kelvin_ring()
+    p4 = T.default_params(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, hyperfriction=0)
+    g4 = k4.copy()
+    ctx.rk4_run(g4, 0.0, 1e-7, 200, p4)
+    c4 = {"points": len(k4.active), "steps": 200, "dt_s": 1e-7, "gpu_fp64_ms": round(ctx.last_device_ms(), 2)}
+    if ref is not None:
+        a4 = k4.copy()
+        ref.set_params(p4)
+        w0 = time.time()
+        for k in range(3):
+            ref.step(a4, k * 1e-7, 1e-7, p4)
+        c4["cpu_reference_ms_for_200_steps"] = round((time.time() - w0) * 1e3 * 200 / 3, 1)
+        c4["cpu_steps_timed"] = 3
+    res["cfg4_kelvin4096"] = c4
+    return res
+
+
 def only_section(args, ctx, T, tg, common):
     """One optional section on its own (--only), printed as one JSON line."""
     t0 = time.time()
@@ -250,6 +325,11 @@ def only_section(args, ctx, T, tg, common):
         res["reconnection_candidates"] = reconnection_section(ctx, tg)
     elif args.only == "ensemble":
         res["ensemble"] = ensemble_section(ctx, T, args)
+    elif args.only == "configs":
+        from oracle.oracle import Reference
+        thr = args.cpu_threads or host_threads()
+        ref = Reference(thr) if Reference.available() else None
+        res["configs"] = configs_section(ctx, T, ref, thr, True)
     else:
         res["adjoint"] = adjoint_section(ctx, T)
     res["wall_s"] = round(time.time() - t0, 1)
@@ -300,8 +380,10 @@ def main():
                     help="Kelvin-ring gradient (4096 points, 200 steps, checkpoints every 20; 0: skip)")
     ap.add_argument("--reconnect", type=int, default=1,
                     help="reconnection candidate search on the workload (0: skip)")
-    ap.add_argument("--only", choices=("octree", "topology", "reconnection", "ensemble", "adjoint"),
+    ap.add_argument("--only", choices=("octree", "topology", "reconnection", "ensemble", "adjoint", "configs"),
                     help="run this section alone on the same workload (no headline steps), for profiles")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="the other configurations beside the reference's own C (0: skip)")
     ap.add_argument("--topology", type=int, default=1,
                     help="device-resident topology edits and the 200-step main loop (0: skip)")
     ap.add_argument("--tree", type=int, default=1,
@@ -519,6 +601,12 @@ def main():
         log(f"topology: {out['topology']}")
     elif args.topology > 0:
         skipped.append("topology")
+
+    if args.configs > 0 and left() > 45:
+        out["configs"] = configs_section(ctx, T, eng if kind == "reference" else None, threads, False)
+        log(f"configs: {out['configs']}")
+    elif args.configs > 0:
+        skipped.append("configs")
 
     if args.ensemble > 0 and left() > 60:
         out["ensemble"] = ensemble_section(ctx, T, args)
