@@ -181,8 +181,8 @@ void *ov_ctx_stream(ov_ctx *ctx);
 
 /* update_tangents_normals (tangle.h:120): writes dxi, tangents, normals.
  *
- * Walls (this call, ov_gpu_velocities and ov_gpu_rk4_step / ov_gpu_rk4_run
- * with the direct sum): PINNED and PINNED_SLIP points with pin_wall 0..5
+ * Walls (this call, ov_gpu_velocities and ov_gpu_rk4_step / ov_gpu_rk4_run,
+ * direct sum or octree): PINNED and PINNED_SLIP points with pin_wall 0..5
  * may end open chains (forward or reverse -1).  Neighbours past a chain end
  * are the mirror images of the chain in that point's wall (step_node,
  * tangle.c:165-198); PINNED points take the one-sided stencil (tangle.c:
@@ -190,7 +190,8 @@ void *ov_ctx_stream(ov_ctx *ctx);
  * tests), their non-PINNED neighbours the neighbour-tangent stencil
  * (tangle.c:364-430); PINNED points get vs = 0 and vels = v_b
  * (tangle.c:589-594), PINNED_SLIP points lose the velocity normal to their
- * wall (tangle.c:677-683).  Refused with OV_E_INVALID: a free point without
+ * wall (tangle.c:677-683); in the octree a chain end has no segment
+ * (octree.c:254-255).  Refused with OV_E_INVALID: a free point without
  * both links, a pinned point without any, two neighbouring points that both
  * border a PINNED point (the reference recurses without end there).  The
  * reference reads status[-1] at a PINNED_SLIP chain end; here that missing

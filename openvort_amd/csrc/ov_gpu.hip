@@ -932,8 +932,6 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
   const long cap = ctx->cap;
   int rc;
   if (n == 0) return OV_OK;
-  if (ctx->has_pins && ctx->use_tree && !ctx->lia_only)
-    return fail(ctx, OV_E_UNSUPPORTED, "points pinned on walls: direct sum only (use_BH = 0)");
   if ((rc = geometry(ctx, X, geom, Tg, Ng, dxi))) return rc;
   int have_bs = 0;
   if (!ctx->lia_only && ctx->use_tree) {
@@ -1475,7 +1473,7 @@ int ix_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
     const int st = t->status[k].status;
     if (st == OV_EMPTY) continue;
     if (st == OV_PINNED || st == OV_PINNED_SLIP)
-      return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are not supported", k);
+      return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: the topology edits take free points only", k);
     if (st != OV_FREE) return fail(ctx, OV_E_INVALID, "bad status %d at %d", st, k);
     const int f = t->connections[k].forward, r = t->connections[k].reverse;
     if (f < 0 || f >= N || r < 0 || r >= N || t->status[f].status != OV_FREE ||

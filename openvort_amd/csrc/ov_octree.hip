@@ -171,7 +171,11 @@ __global__ void __launch_bounds__(256) k_gather_u64(const uint64_t* __restrict__
 }
 
 // Sorted position j -> stream point, and the point's segment (the segment
-// of octree cells, octree.c:253-258 and tangle.c:519-528).
+// of octree cells, octree.c:253-258 and tangle.c:519-528).  A point without
+// a forward link (the pinned end of an open chain) has none: its entry is
+// the zero-length segment a -> a, which adds exactly nothing to the cell
+// moments (the reference skips it, octree.c:254-255) and whose field is
+// exactly zero by the collinearity rule (segment_field returns 0 for it).
 __global__ void __launch_bounds__(256) k_oct_leafsegs(const double* __restrict__ P, long cap,
                                                       const int* __restrict__ fwd,
                                                       const int* __restrict__ order,
@@ -184,11 +188,17 @@ __global__ void __launch_bounds__(256) k_oct_leafsegs(const double* __restrict__
   sorted_s[j] = s;
   LeafSeg q;
   const int f = fwd[s];
-  const double pf[3] = {P[f], P[cap + f], P[2 * cap + f]};
   q.a[0] = P[s];
   q.a[1] = P[cap + s];
   q.a[2] = P[2 * cap + s];
-  pwrap(q.a, pf, box, q.b);
+  if (f >= 0) {
+    const double pf[3] = {P[f], P[cap + f], P[2 * cap + f]};
+    pwrap(q.a, pf, box, q.b);
+  } else {
+    q.b[0] = q.a[0];
+    q.b[1] = q.a[1];
+    q.b[2] = q.a[2];
+  }
   q.m = s;
   q.fwd = f;
   segs[j] = q;

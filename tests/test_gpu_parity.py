@@ -167,9 +167,9 @@ def test_errors(ctx):
         ctx.velocities(bad, 0.0, p)
     assert e.value.code == OV_E_INVALID
     bad = t.copy()
-    bad.status[3] = (T.PINNED, T.Z_L)    # walls: direct sum only
+    bad.status[3] = (T.PINNED, T.Z_L)    # walls: not in the ensemble path
     with pytest.raises(OVError) as e:
-        ctx.velocities(bad, 0.0, ring_params(use_BH=1))
+        ctx.rk4_run_batch([bad], 0.0, 1e-5, 1, p)
     assert e.value.code == OV_E_UNSUPPORTED
     bad = t.copy()
     bad.connections[5, 0] = -1           # a free point must have both links

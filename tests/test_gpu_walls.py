@@ -222,10 +222,28 @@ def test_wall_rk4_matches_reference(ctx):
         np.testing.assert_array_equal(b.vnodes[ends, 2], t.vnodes[ends, 2])
 
 
+@pytest.mark.parametrize("theta", [0.3, 0.6])
+def test_wall_octree_matches_reference(ctx, theta):
+    """The octree with lines pinned on the walls against the reference's own
+    octree: chain ends carry no segment (octree.c:254-255, tangle.c:519-528),
+    points on the walls stay in the cells that contain them."""
+    t = wall_tangle(status=T.PINNED, holes=17)
+    p = params(use_BH=1, BH_resolution=theta)
+    a, b = copy(t), copy(t)
+    R = Reference()
+    R.set_params(p)
+    R.update_tangle(a, 0.0, p)
+    ctx.velocities(b, 0.0, p)
+    act = a.active
+    e = rel_err(b.vels[act], a.vels[act])
+    print(f"theta {theta}: vels {e}", flush=True)
+    assert e[0] < 1e-10
+
+
 def test_wall_unsupported_paths(ctx):
     t = wall_tangle()
-    for fn in (lambda: ctx.velocities(copy(t), 0.0, params(use_BH=1)),
-               lambda: ctx.rk4_run_batch([copy(t)], 0.0, 1e-5, 1, params())):
+    for fn in (lambda: ctx.rk4_run_batch([copy(t)], 0.0, 1e-5, 1, params()),
+               lambda: ctx.reconnection_candidates(copy(t), 1e-3, 0.087)):
         with pytest.raises(OVError) as e:
             fn()
         assert e.value.code == OV_E_UNSUPPORTED
