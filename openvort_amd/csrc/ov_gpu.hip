@@ -115,6 +115,7 @@ struct ov_ctx {
   long* t_stats = nullptr;
   // points pinned on walls (stream order): pin code per point, see ov_walls.hip
   bool has_pins = false;
+  int n_seg = 0;                  // stream points with a forward link
   std::vector<int> pin_s;
   int* pin = nullptr;
   long pin_cap = 0;
@@ -529,6 +530,8 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p, bool allo
   }
   ctx->act.swap(act_s);
   ctx->n = n;
+  ctx->n_seg = n;
+  for (int q = 0; q < n; ++q) ctx->n_seg -= ctx->fwd_s[q] < 0 ? 1 : 0;
 
   // box, images and physics
   ImgDev il[MAX_IMAGES];
@@ -1002,15 +1005,16 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
                      (int)chunks, n, cap, ctx->bs)))
       return rc;
     have_bs = 1;
-    double nd = (double)n;
-    ctx->last_interactions = nd * (nd - 2) + (double)(n_img - 1) * nd * nd;
+    // N_tgt (N_seg - 2) + n_img N_tgt N_seg with N_seg the points that start a segment
+    const double nd = (double)n, ns = (double)ctx->n_seg;
+    ctx->last_interactions = nd * (ns - 2) + (double)(n_img - 1) * nd * ns;
   } else {
     ctx->last_interactions = 0;
   }
   if ((rc = launch(ctx, "k_epilogue", k_epilogue, grid1(n), dim3(256), X, (const double*)Tg,
                    (const double*)Ng, (const double*)ctx->lia, (const double*)ctx->bs, have_bs, n,
-                   cap, time, ctx->ph, ctx->has_pins ? (const int*)ctx->pin : (const int*)nullptr, vs, vel,
-                   ctx->flag)))
+                   cap, time, ctx->ph,
+                   ctx->has_pins ? (const int*)ctx->pin : (const int*)nullptr, vs, vel, ctx->flag)))
     return rc;
   return OV_OK;
 }
@@ -1707,6 +1711,7 @@ int ix_to_stream(ov_ctx* ctx) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   const int n = last[0] + last[1];
   ctx->n = n;
+  ctx->n_seg = n;   // closed loops only
   if ((rc = ensure_capacity(ctx, std::max(n, 1)))) return rc;
   ctx->cap = ctx->buf_cap;
   if ((rc = launch(ctx, "k_stream_pos", k_stream_pos, grid1(N), dim3(256), (const int*)ctx->ix_lab[0],
@@ -2476,6 +2481,7 @@ int ov_gpu_velocities_noalloc(ov_ctx* ctx, const double* pos, const int* fwd, co
   }
   if (v.use_tree) return fail(ctx, OV_E_UNSUPPORTED, "noalloc: direct sum only (use_BH = 0)");
   v.n = n;
+  v.n_seg = n;
   v.cap = stride;
   v.max_chunks = NOALLOC_CHUNKS;
   v.profile = false;

@@ -204,6 +204,18 @@ __device__ __forceinline__ int block_exscan256(int v, int* lds, int* total) {
 // pinned end of an open chain) has no segment: brk = 2, no second record.
 // Per block: exclusive scan of the second-record flags (rank[s]) and the
 // block total.
+__device__ __forceinline__ int cont_flag(const double* __restrict__ P, const int* __restrict__ fwd,
+                                         int s, long cap, const BoxDev& box) {
+  const int f = fwd[s];
+  if (f < 0) return 2;
+  double a[3], pf[3], b[3];
+  load3(P, cap, s, a);
+  load3(P, cap, f, pf);
+  pwrap(a, pf, box, b);
+  const bool cont = (f == s + 1) && b[0] == pf[0] && b[1] == pf[1] && b[2] == pf[2];
+  return cont ? 0 : 1;
+}
+
 __global__ void __launch_bounds__(256) k_cont(const double* __restrict__ P,
                                               const int* __restrict__ fwd, int n, long cap,
                                               BoxDev box, int* __restrict__ brk,
@@ -212,18 +224,9 @@ __global__ void __launch_bounds__(256) k_cont(const double* __restrict__ P,
   const int s = blockIdx.x * 256 + threadIdx.x;
   int nc = 0;
   if (s < n) {
-    int f = fwd[s];
-    if (f < 0) {
-      brk[s] = 2;
-    } else {
-      double a[3], pf[3], b[3];
-      load3(P, cap, s, a);
-      load3(P, cap, f, pf);
-      pwrap(a, pf, box, b);
-      bool cont = (f == s + 1) && b[0] == pf[0] && b[1] == pf[1] && b[2] == pf[2];
-      nc = cont ? 0 : 1;
-      brk[s] = nc;
-    }
+    const int b = cont_flag(P, fwd, s, cap, box);
+    brk[s] = b;
+    nc = b == 1 ? 1 : 0;
   }
   int tot;
   int r = block_exscan256(nc, lds, &tot);
@@ -270,18 +273,13 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ bsum, in
   }
 }
 
-// Records of point s at offset s + (breaks before s): the point itself,
-// closing segment s-1 when that one continues, then, for a break, the end
-// of segment s (the forward point's minimum image).
-__global__ void __launch_bounds__(256) k_records(const double* __restrict__ P,
-                                                 const int* __restrict__ fwd, int n, long cap,
-                                                 BoxDev box, const int* __restrict__ brk,
-                                                 const int* __restrict__ rank,
-                                                 const int* __restrict__ bsum,
-                                                 NearRec* __restrict__ near) {
-  const int s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= n) return;
-  const int off = s + bsum[blockIdx.x] + rank[s];
+// Records of point s at offset off = s + (breaks before s): the point
+// itself, closing segment s-1 when that one continues, then, for a break,
+// the end of segment s (the forward point's minimum image).
+__device__ __forceinline__ void point_records(const double* __restrict__ P, const int* __restrict__ fwd,
+                                              int s, long cap, const BoxDev& box,
+                                              const int* __restrict__ brk, int off,
+                                              NearRec* __restrict__ near) {
   double a[3];
   load3(P, cap, s, a);
   NearRec r0;
@@ -311,18 +309,23 @@ __global__ void __launch_bounds__(256) k_records(const double* __restrict__ P,
   }
 }
 
+__global__ void __launch_bounds__(256) k_records(const double* __restrict__ P,
+                                                 const int* __restrict__ fwd, int n, long cap,
+                                                 BoxDev box, const int* __restrict__ brk,
+                                                 const int* __restrict__ rank,
+                                                 const int* __restrict__ bsum,
+                                                 NearRec* __restrict__ near) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  point_records(P, fwd, s, cap, box, brk, s + bsum[blockIdx.x] + rank[s], near);
+}
+
 // One 64-thread block per source tile: frame centre = centre of the bounding
 // box of the tile's records and of the record before the tile; FP32 records
 // in that frame with the FP64 segment vector rounded once; padding records
 // beyond M (unlinked, at the centre) complete the last tile.
-__global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
-                                                  const int* __restrict__ counts,
-                                                  SrcTile* __restrict__ tiles,
-                                                  FarRec* __restrict__ far) {
-  const int M = counts[0], nst = counts[1];
-  const int J = blockIdx.x;
-  if (J >= nst) return;
-  const int lane = threadIdx.x;
+__device__ __forceinline__ void source_tile(NearRec* __restrict__ near, int M, int J, int lane,
+                                            SrcTile* __restrict__ tiles, FarRec* __restrict__ far) {
   const int e = J * TILE + lane;
   const bool valid = e < M;
   NearRec r;
@@ -390,11 +393,20 @@ __global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
   }
 }
 
+__global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
+                                                  const int* __restrict__ counts,
+                                                  SrcTile* __restrict__ tiles,
+                                                  FarRec* __restrict__ far) {
+  const int M = counts[0], nst = counts[1];
+  const int J = blockIdx.x;
+  if (J >= nst) return;
+  source_tile(near, M, J, threadIdx.x, tiles, far);
+}
+
 // Bounding box of each 64-point target tile.
-__global__ void __launch_bounds__(64) k_tgt_tiles(const double* __restrict__ P, int n, long cap,
-                                                  TgtTile* __restrict__ tiles) {
-  const int T = blockIdx.x;
-  const int s = T * TILE + threadIdx.x;
+__device__ __forceinline__ void target_tile(const double* __restrict__ P, int n, long cap, int T, int lane,
+                                            TgtTile* __restrict__ tiles) {
+  const int s = T * TILE + lane;
   double r[3];
   if (s < n) {
     load3(P, cap, s, r);
@@ -407,7 +419,12 @@ __global__ void __launch_bounds__(64) k_tgt_tiles(const double* __restrict__ P, 
     t.lo[i] = wave_min(r[i]);
     t.hi[i] = wave_max(r[i]);
   }
-  if (threadIdx.x == 0) tiles[T] = t;
+  if (lane == 0) tiles[T] = t;
+}
+
+__global__ void __launch_bounds__(64) k_tgt_tiles(const double* __restrict__ P, int n, long cap,
+                                                  TgtTile* __restrict__ tiles) {
+  target_tile(P, n, cap, blockIdx.x, threadIdx.x, tiles);
 }
 
 // For each target, the records closing its two adjacent segments (the one
