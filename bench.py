@@ -494,8 +494,8 @@ def main():
     kd_avg_s = kd_ms / max(n_launch, 1) / 1e3
     achieved = FLOP_PER_INTERACTION * I_eval / kd_avg_s
     kernels = {}
-    for k in ("k_geometry", "k_cont", "k_scan_blocks", "k_records", "k_src_tiles", "k_tgt_tiles",
-              "k_adjacent", "k_direct_mixed", "k_reduce", "k_epilogue", "k_stage", "k_combine"):
+    for k in ("k_geometry", "k_cont", "k_scan_blocks", "k_records", "k_src_tiles",
+              "k_direct_mixed", "k_reduce", "k_epilogue", "k_stage", "k_combine"):
         nl, ms = ctx.kernel_stats(k)
         if nl:
             kernels[k] = {"launches": nl, "avg_ms": round(ms / nl, 4)}

@@ -946,8 +946,9 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
     const int nb = (n + 255) / 256;
     const int max_src = (2 * n + TILE - 1) / TILE;
     const int n_img = ctx->n_img;
+    // also the target tiles' boxes and, for open chains, adjacency entries of -1
     if ((rc = launch(ctx, "k_cont", k_cont, dim3(nb), dim3(256), X, (const int*)ctx->fwd, n, cap,
-                     ctx->box, ctx->brk, ctx->rank, ctx->bsum)))
+                     ctx->box, ctx->brk, ctx->rank, ctx->bsum, ctx->tgt, ctx->has_pins ? 1 : 0, ctx->adj)))
       return rc;
     if ((rc = launch(ctx, "k_scan_blocks", k_scan_blocks, dim3(1), dim3(1024), ctx->bsum, nb, n,
                      ctx->counts)))
@@ -956,17 +957,9 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
                      cap, ctx->box, (const int*)ctx->brk, (const int*)ctx->rank,
                      (const int*)ctx->bsum, ctx->near)))
       return rc;
+    // also the adjacency of each target's two segments
     if ((rc = launch(ctx, "k_src_tiles", k_src_tiles, dim3(max_src), dim3(TILE), ctx->near,
-                     (const int*)ctx->counts, ctx->src, ctx->far)))
-      return rc;
-    if ((rc = launch(ctx, "k_tgt_tiles", k_tgt_tiles, dim3(n_tgt), dim3(TILE), X, n, cap, ctx->tgt)))
-      return rc;
-    // an open chain's end has no segment: its adjacency entry stays -1
-    if (ctx->has_pins &&
-        (rc = launch(ctx, "k_fill_int", k_fill_int, grid1(2 * n), dim3(256), ctx->adj, 2 * n, -1)))
-      return rc;
-    if ((rc = launch(ctx, "k_adjacent", k_adjacent, dim3((unsigned)((2 * n + 255) / 256)), dim3(256),
-                     (const NearRec*)ctx->near, (const int*)ctx->counts, ctx->adj)))
+                     (const int*)ctx->counts, ctx->src, ctx->far, ctx->adj)))
       return rc;
     // enough (target tile, chunk) waves to fill 256 CUs several times over,
     // but no more chunks than work items per target tile (each chunk's

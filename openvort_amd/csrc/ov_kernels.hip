@@ -203,7 +203,28 @@ __device__ __forceinline__ int block_exscan256(int v, int* lds, int* total) {
 // the point emits a second record.  A point without a forward link (the
 // pinned end of an open chain) has no segment: brk = 2, no second record.
 // Per block: exclusive scan of the second-record flags (rank[s]) and the
-// block total.
+// block total.  Each wavefront also writes the bounding box of its 64-point
+// target tile, and with fill_adj the adjacency entries of its points start
+// at -1 (open chains leave some of them unset).
+// Bounding box of each 64-point target tile.
+__device__ __forceinline__ void target_tile(const double* __restrict__ P, int n, long cap, int T, int lane,
+                                            TgtTile* __restrict__ tiles) {
+  const int s = T * TILE + lane;
+  double r[3];
+  if (s < n) {
+    load3(P, cap, s, r);
+  } else {
+    load3(P, cap, T * TILE, r);
+  }
+  TgtTile t;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    t.lo[i] = wave_min(r[i]);
+    t.hi[i] = wave_max(r[i]);
+  }
+  if (lane == 0) tiles[T] = t;
+}
+
 __device__ __forceinline__ int cont_flag(const double* __restrict__ P, const int* __restrict__ fwd,
                                          int s, long cap, const BoxDev& box) {
   const int f = fwd[s];
@@ -219,9 +240,17 @@ __device__ __forceinline__ int cont_flag(const double* __restrict__ P, const int
 __global__ void __launch_bounds__(256) k_cont(const double* __restrict__ P,
                                               const int* __restrict__ fwd, int n, long cap,
                                               BoxDev box, int* __restrict__ brk,
-                                              int* __restrict__ rank, int* __restrict__ bsum) {
+                                              int* __restrict__ rank, int* __restrict__ bsum,
+                                              TgtTile* __restrict__ tgt, int fill_adj,
+                                              int* __restrict__ adj) {
   __shared__ int lds[4];
   const int s = blockIdx.x * 256 + threadIdx.x;
+  const int T = __builtin_amdgcn_readfirstlane(s >> 6);
+  if (T * TILE < n) target_tile(P, n, cap, T, threadIdx.x & 63, tgt);
+  if (fill_adj && s < n) {
+    adj[2 * s] = -1;
+    adj[2 * s + 1] = -1;
+  }
   int nc = 0;
   if (s < n) {
     const int b = cont_flag(P, fwd, s, cap, box);
@@ -393,54 +422,30 @@ __device__ __forceinline__ void source_tile(NearRec* __restrict__ near, int M, i
   }
 }
 
+// One 64-thread block per source tile (source_tile), and for each target
+// the records closing its two adjacent segments (the one that starts at it
+// and the one that ends at it, reference tangle.c:612-615): adj[2 s] and
+// adj[2 s + 1].  A wrapped adjacent segment lies a box length away in the
+// primary frame, so the sum must find it wherever it is.
 __global__ void __launch_bounds__(64) k_src_tiles(NearRec* __restrict__ near,
                                                   const int* __restrict__ counts,
                                                   SrcTile* __restrict__ tiles,
-                                                  FarRec* __restrict__ far) {
+                                                  FarRec* __restrict__ far, int* __restrict__ adj) {
   const int M = counts[0], nst = counts[1];
   const int J = blockIdx.x;
   if (J >= nst) return;
+  const int e = J * TILE + threadIdx.x;
+  if (e < M) {
+    const NearRec r = near[e];
+    if (r.seg >= 0) {
+      adj[2 * r.seg] = e;
+      adj[2 * r.seg_fwd + 1] = e;
+    }
+  }
   source_tile(near, M, J, threadIdx.x, tiles, far);
 }
 
-// Bounding box of each 64-point target tile.
-__device__ __forceinline__ void target_tile(const double* __restrict__ P, int n, long cap, int T, int lane,
-                                            TgtTile* __restrict__ tiles) {
-  const int s = T * TILE + lane;
-  double r[3];
-  if (s < n) {
-    load3(P, cap, s, r);
-  } else {
-    load3(P, cap, T * TILE, r);
-  }
-  TgtTile t;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    t.lo[i] = wave_min(r[i]);
-    t.hi[i] = wave_max(r[i]);
-  }
-  if (lane == 0) tiles[T] = t;
-}
 
-__global__ void __launch_bounds__(64) k_tgt_tiles(const double* __restrict__ P, int n, long cap,
-                                                  TgtTile* __restrict__ tiles) {
-  target_tile(P, n, cap, blockIdx.x, threadIdx.x, tiles);
-}
-
-// For each target, the records closing its two adjacent segments (the one
-// that starts at it and the one that ends at it, reference tangle.c:612-615):
-// adj[2 s] and adj[2 s + 1].  A wrapped adjacent segment lies a box length
-// away in the primary frame, so the sum must find it wherever it is.
-__global__ void __launch_bounds__(256) k_adjacent(const NearRec* __restrict__ near,
-                                                  const int* __restrict__ counts,
-                                                  int* __restrict__ adj) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= counts[0]) return;
-  const NearRec r = near[e];
-  if (r.seg < 0) return;
-  adj[2 * r.seg] = e;
-  adj[2 * r.seg_fwd + 1] = e;
-}
 
 // ---------------------------------------------------------------------------
 // direct Biot-Savart sum over all segments and images
