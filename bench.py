@@ -656,7 +656,8 @@ def topology_section(ctx, T, tg, ref, threads):
     eliminate_small_loops, enforce_boundaries) for 200 steps of the 64-ring
     tangle with v_n = 0.5 cm/s along x and dt = 5e-6 s, dl_min / dl_max =
     1.2e-4 / 2.4e-4 cm, mixed direct sum, resident on the device
-    (ov_gpu_run)."""
+    (ov_gpu_run).  (d) The same loop on the 1M-point tangle with the octree
+    (theta 0.5, FP32 multipoles), 5 steps at dt = 1e-6 s."""
     res = {}
     x = jittered(tg, 1e-4, 0.15, 5, tg.N + (tg.N * 3) // 20)
     ctx.update_tangents_normals(x)
@@ -727,6 +728,22 @@ def topology_section(ctx, T, tg, ref, threads):
         "topology_kernels_ms_per_step": round(topo_ms / 20, 3),
         "k_remesh_ms_per_step": round(ctx.kernel_stats("k_remesh")[1] / 20, 3),
         "k_direct_mixed_ms_per_step": round(ctx.kernel_stats("k_direct_mixed")[1] / 20, 3)}
+
+    # (d) the main loop on the 1M-point tangle with the octree (theta 0.5,
+    # FP32 multipoles), dt 1e-6 s, dl_min / dl_max = 0.6 / 1.2 x spacing
+    x = tg.expand(tg.N + tg.N // 20)
+    pb = T.default_params(vortex_width=1.3e-8, alpha=0.098, alpha_p=0.016, precision=T.PREC_MIXED,
+                          use_BH=1, BH_resolution=0.5, dl_max=1.2e-4)
+    tpb = T.default_topology(dl_min=0.6e-4, dl_max=1.2e-4)
+    ctx.update_tangents_normals(x)
+    n_start = int(len(x.active))
+    ctx.reset_stats()
+    ctx.run(x, 0.0, 1e-6, 5, pb, tpb, profile=True)
+    res["main_loop_1m_octree"] = {
+        "steps": 5, "dt_s": 1e-6, "theta": 0.5, "points_start": n_start, "points_end": int(len(x.active)),
+        "device_ms_per_step": round(ctx.last_device_ms() / 5, 2),
+        "k_octree_mixed_ms_per_step": round(ctx.kernel_stats("k_octree_mixed")[1] / 5, 2),
+        "topology_kernels_ms_per_step": round(sum(ctx.kernel_stats(k)[1] for k in TOPO_KERNELS) / 5, 2)}
     return res
 
 

@@ -329,7 +329,9 @@ int ov_gpu_enforce_boundaries(ov_ctx *ctx, ov_tangle_state *tangle);
  * after the last step's edits, and vels / vs of the last step's k1 (slots
  * created afterwards keep the values the slot held, as in the reference).
  * OV_E_NOMEM as ov_gpu_remesh when the capacity runs out (nothing written).
- * Direct sum only.  Flags: OV_PROFILE. */
+ * Direct sum or octree (use_BH: each evaluation rebuilds the tree, points
+ * inserted in ascending index as the reference inserts them).  Flags:
+ * OV_PROFILE. */
 int ov_gpu_run(ov_ctx *ctx, ov_tangle_state *tangle, double t0, double dt, int nsteps,
                const ov_params *params, const ov_topology *tp, int flags);
 

@@ -119,6 +119,7 @@ struct ov_ctx {
   std::vector<int> pin_s;
   int* pin = nullptr;
   long pin_cap = 0;
+  bool tree_order_device = false;   // t_order already set on the device (ov_gpu_run)
   int oct_images_per_walk = 9;   // most images per traversal wavefront (64 / 9 = 7 targets)
   int* t_big = nullptr;
   long t_big_cap = 0;
@@ -408,6 +409,7 @@ int setup_box_physics(ov_ctx* ctx, const ov_domain_box* bx, const ov_boundary_im
 // point it creates has none, tangle.c:94-99, 1063-1067).
 int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p, bool allow_pins = false) {
   int rc = 0;
+  ctx->tree_order_device = false;
   if (!t || !p) return fail(ctx, OV_E_INVALID, "null tangle or params");
   if (t->N < 0) return fail(ctx, OV_E_INVALID, "negative N");
   if (t->N > 0 && (!t->vnodes || !t->connections || !t->status))
@@ -714,8 +716,9 @@ int build_tree(ov_ctx* ctx, const double* X) {
   const long cap = ctx->cap;
   int rc;
   if ((rc = tree_point_buffers(ctx, n))) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->t_order, ctx->pos_by_index.data(), sizeof(int) * n,
-                        hipMemcpyHostToDevice, ctx->stream));
+  if (!ctx->tree_order_device)
+    HIPCHK(hipMemcpyAsync(ctx->t_order, ctx->pos_by_index.data(), sizeof(int) * n,
+                          hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemsetAsync(ctx->t_maxd, 0, sizeof(int), ctx->stream));
   const dim3 gp((unsigned)((n + 255) / 256));
   if ((rc = launch(ctx, "k_oct_keys", k_oct_keys, gp, dim3(256), X, cap, (const int*)ctx->t_order, n,
@@ -1462,6 +1465,7 @@ int ix_buffers(ov_ctx* ctx, int N) {
 // consistent links; pinned points are not supported) and set box/physics.
 int ix_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   if (!t) return fail(ctx, OV_E_INVALID, "null tangle");
+  ctx->tree_order_device = false;
   const int N = t->N;
   if (N < 0) return fail(ctx, OV_E_INVALID, "negative N");
   if (N > 0 && (!t->vnodes || !t->connections || !t->status))
@@ -1715,6 +1719,26 @@ int ix_to_stream(ov_ctx* ctx) {
                 (long)ctx->ix_alloc, (const int*)ctx->ix_fwd, (const int*)ctx->ix_rev,
                 (const int*)ctx->ix_act, (const int*)ctx->ix_pos, n, ctx->cap, ctx->X0, ctx->fwd,
                 ctx->rev);
+}
+
+// The octree's insertion order of the device-built stream (what prepare()'s
+// pos_by_index holds for a host-built one), into t_order.
+int ix_tree_order(ov_ctx* ctx) {
+  const int N = ctx->ix_N;
+  int rc;
+  if ((rc = tree_point_buffers(ctx, ctx->n))) return rc;
+  if ((rc = launch(ctx, "k_active_flags", k_active_flags, grid1(N), dim3(256), (const int*)ctx->ix_pos, N,
+                   ctx->ix_bits)))
+    return rc;
+  size_t sb = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, ctx->ix_bits, ctx->ix_all, N, ctx->stream));
+  if ((rc = cub_temp(ctx, sb))) return rc;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->t_cub, sb, ctx->ix_bits, ctx->ix_all, N, ctx->stream));
+  if ((rc = launch(ctx, "k_tree_order", k_tree_order, grid1(N), dim3(256), (const int*)ctx->ix_pos,
+                   (const int*)ctx->ix_all, N, ctx->t_order)))
+    return rc;
+  ctx->tree_order_device = true;
+  return OV_OK;
 }
 
 int ix_scatter3(ov_ctx* ctx, const double* src, double* dst) {
@@ -2586,7 +2610,6 @@ int ov_gpu_run(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps
   if (!p || !tp) return fail(ctx, OV_E_INVALID, "null params or topology settings");
   if (nsteps < 0) return fail(ctx, OV_E_INVALID, "negative step count");
   if (flags & OV_USE_CALLER_K1) return fail(ctx, OV_E_INVALID, "ov_gpu_run evaluates k1 itself");
-  if (p->use_BH) return fail(ctx, OV_E_UNSUPPORTED, "ov_gpu_run: direct sum only (use_BH = 0)");
   HIPCHK(hipSetDevice(ctx->device));
   int rc = ix_prepare(ctx, t, p);
   if (rc) return rc;
@@ -2608,6 +2631,7 @@ int ov_gpu_run(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps
     // remesh; eliminate_small_loops; enforce_boundaries
     if ((rc = ix_to_stream(ctx))) return rc;
     if (ctx->n == 0) break;
+    if (ctx->use_tree && (rc = ix_tree_order(ctx))) return rc;
     if ((rc = rk4_device(ctx, time, dt, p->rk_variant, false))) return rc;
     if ((rc = ix_scatter3(ctx, ctx->K[0], ctx->ix_vels)) || (rc = ix_scatter3(ctx, ctx->vs1, ctx->ix_vs)))
       return rc;

@@ -492,6 +492,21 @@ __global__ void __launch_bounds__(256) k_stream_pos(const int* __restrict__ lab,
   pos[k] = s;
   act[s] = k;
 }
+// The octree's insertion order (points in ascending caller index, as the
+// reference inserts them) as stream positions: flag[k] = 1 for an active
+// index, then order[scan[k]] = pos[k] after an exclusive scan of the flags.
+__global__ void __launch_bounds__(256) k_active_flags(const int* __restrict__ pos, int N,
+                                                      int* __restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) flag[k] = pos[k] >= 0 ? 1 : 0;
+}
+__global__ void __launch_bounds__(256) k_tree_order(const int* __restrict__ pos,
+                                                    const int* __restrict__ scan, int N,
+                                                    int* __restrict__ order) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N && pos[k] >= 0) order[scan[k]] = pos[k];
+}
+
 // Stream-order positions and links gathered from the index space.
 __global__ void __launch_bounds__(256) k_gather_stream(const double* __restrict__ Xi, long icap,
                                                        const int* __restrict__ fwd,

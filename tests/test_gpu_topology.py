@@ -140,14 +140,15 @@ def test_enforce_boundaries_matches_reference(ctx):
     np.testing.assert_array_equal(b.vnodes[act], a.vnodes[act])
 
 
-@pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
-def test_run_matches_host_sequence(ctx, prec):
+@pytest.mark.parametrize("prec,bh", [(T.PREC_FP64, 0), (T.PREC_MIXED, 0), (T.PREC_FP64, 1), (T.PREC_MIXED, 1)])
+def test_run_matches_host_sequence(ctx, prec, bh):
     """ov_gpu_run over 4 steps equals the same steps driven from the host
     with the single-call entry points (ov_gpu_rk4_step,
     ov_gpu_update_tangents_normals, ov_gpu_remesh,
     ov_gpu_eliminate_small_loops, ov_gpu_enforce_boundaries) bit for bit:
-    the device rebuilds the point stream exactly as the host does."""
-    p = params(precision=prec, near_radius=0.01)
+    the device rebuilds the point stream exactly as the host does, and with
+    the octree (bh) the tree's insertion order as well."""
+    p = params(precision=prec, near_radius=0.01, use_BH=bh, BH_resolution=0.5)
     tp = T.default_topology(dl_min=1e-3, dl_max=2e-3)
     t = uneven_tangle(seed=21)
     a, b = t.copy(), t.copy()
