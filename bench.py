@@ -466,7 +466,10 @@ def main():
     # ---- RK4 steps: at most one warm-up step, then timed steps while the
     # budget holds (a step is about four evaluations) -------------------------
     est_step_s = 4.2 * mx_ms / 1e3 + 0.5
-    reserve_s = 40.0 + (30.0 if args.cpu_sample > 0 else 0.0)   # CPU leg + print
+    # CPU leg + print, and the optional sections' measured ~60 s (octree, the
+    # reference's octree, reconnection, ensemble, adjoint, topology, configs)
+    optional = (args.tree, args.reconnect, args.ensemble, args.adjoint, args.topology, args.configs)
+    reserve_s = 40.0 + (30.0 if args.cpu_sample > 0 else 0.0) + (70.0 if any(optional) else 0.0)
     warm = 0
     x = tg.copy()
     t_sim = 0.0
