@@ -42,7 +42,8 @@ struct OctCell {                   // build-time cell, breadth-first order
 struct alignas(64) OctNode {
   double com[3];
   double lmax, lmin;
-  int n, first, skip, pad;
+  int n, first, skip;
+  float q;   // (lmax / theta)^2 rounded to FP32: the FP32 opening test's threshold on d^2
 };
 struct OctMom {
   double gam[3];
@@ -481,14 +482,15 @@ __global__ void __launch_bounds__(256) k_oct_nodes(const OctCell* __restrict__ c
                                                    const int* __restrict__ size,
                                                    const int* __restrict__ pre,
                                                    OctNode* __restrict__ nodes,
-                                                   OctMom* __restrict__ nmom) {
+                                                   OctMom* __restrict__ nmom, double theta) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c > ncell) return;
   OctNode nd;
   if (c == ncell) {   // past the last node: read by the walk's look-ahead only
     nd.com[0] = nd.com[1] = nd.com[2] = 0;
     nd.lmax = nd.lmin = 0;
-    nd.n = nd.first = nd.pad = 0;
+    nd.n = nd.first = 0;
+    nd.q = 0.f;
     nd.skip = 1;
     nodes[ncell] = nd;
     return;
@@ -506,7 +508,8 @@ __global__ void __launch_bounds__(256) k_oct_nodes(const OctCell* __restrict__ c
   nd.n = cell.count;
   nd.first = cell.start;
   nd.skip = size[c];
-  nd.pad = 0;
+  // theta = 0 gives +inf: never accepted, as lmax / d < 0 never holds
+  nd.q = (float)(nd.lmax * nd.lmax / (theta * theta));
   nodes[pre[c]] = nd;
   nmom[pre[c]] = mm;
 }
@@ -637,6 +640,55 @@ __device__ __forceinline__ bool oct_accept(const double dv[3], double lmax, doub
   return d > 0 && lmax / d < theta;
 }
 
+// The opening test in FP32, decided only away from its threshold.  With
+// d = c - rs rounded once from FP64 and wrapped by +-L on periodic axes where
+// |d| > L/2 (d^2 is continuous across that switch, so an FP32 wrap decision
+// that differs from the FP64 one near |d| = L/2 moves d^2 by rounding errors
+// only), each wrapped component e_a is within 2^-23 (|e_a| + L_a) of the
+// FP64 value, so d^2 = sum e_a^2 is within B = 1e-5 d^2 + 5e-7 sum |e_a| L_a
+// (both terms with a margin over the rounding bounds): d^2 - B > q means
+// lmax / d < theta (accept) and d^2 + B < q means it does not (open); in
+// between the FP64 test of oct_accept decides.  q = (lmax / theta)^2 per
+// node.  The test's constants sit in VGPRs (taken from the lane's own
+// position, so the compiler keeps them there) and its FP32 instructions
+// dual-issue, where the FP64 test's do not.
+struct OctFast {
+  float h[3], nh[3], L[3], nL[3];   // +-L/2 and +-L on periodic axes; inf, -inf and 0 elsewhere
+  float eps;                        // 1e-5
+  float ceps;                       // 5e-7
+};
+__device__ __forceinline__ OctFast oct_fast_consts(const BoxDev& b, float lane_zero) {
+  OctFast F;
+  for (int a = 0; a < 3; ++a) {
+    const bool p = b.periodic[a] != 0;
+    F.h[a] = (p ? (float)(b.L[a] / 2) : __builtin_inff()) + lane_zero;
+    F.nh[a] = -F.h[a];
+    F.L[a] = (p ? (float)b.L[a] : 0.f) + lane_zero;
+    F.nL[a] = -F.L[a];
+  }
+  F.eps = 1e-5f + lane_zero;
+  F.ceps = 5e-7f + lane_zero;
+  return F;
+}
+// 1: accept, 0: open, -1: within the band (the FP64 test decides)
+__device__ __forceinline__ int oct_accept32(const double rs[3], const double c[3], float q,
+                                            const OctFast& F) {
+  float e[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float d = (float)(c[a] - rs[a]);
+    float k = d > F.h[a] ? F.nL[a] : 0.f;
+    k = d < F.nh[a] ? F.L[a] : k;
+    e[a] = d + k;
+  }
+  const float d2 = fmaf(e[2], e[2], fmaf(e[1], e[1], e[0] * e[0]));
+  const float s = fmaf(fabsf(e[2]), F.L[2], fmaf(fabsf(e[1]), F.L[1], fabsf(e[0]) * F.L[0]));
+  const float B = fmaf(d2, F.eps, s * F.ceps);
+  if (d2 - B > q) return 1;
+  if (d2 + B < q) return 0;
+  return -1;
+}
+
 constexpr int OCT_WAVES = 4;   // wavefronts per block
 
 // Target of image (sh, w2, ax) for the point r (reference shifted(),
@@ -709,6 +761,9 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
   oct_image_target(r, im.shift, im.wall2, im.axis, rs);
   const int skip_id = (valid && im.primary) ? s : -1;
   const bool dipole = A.op.dipole != 0;
+  // 0 in every lane, but a per-lane value to the compiler
+  const float lane_zero = (float)(lane >> 6);
+  const OctFast F = oct_fast_consts(A.op.box, lane_zero);
   double acc[3] = {0, 0, 0};
   unsigned leaf_pairs = 0, mps = 0, visits = 0;
   int next = valid ? 0 : 0x7fffffff;   // this lane resumes at node >= next
@@ -736,9 +791,13 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
         }
         next = after;
       } else {
-        double dv[3];
-        oct_min_image(rs, nd.com, A.op.box, dv);
-        if (oct_accept(dv, nd.lmax, A.op.theta)) {
+        int dec = oct_accept32(rs, nd.com, nd.q, F);
+        if (__builtin_expect(dec < 0, 0)) {
+          double dv[3];
+          oct_min_image(rs, nd.com, A.op.box, dv);
+          dec = oct_accept(dv, nd.lmax, A.op.theta) ? 1 : 0;
+        }
+        if (dec) {
           const OctMom mm = A.nmom[node];
           if (MIXED)
             oct_multipole32(nd.com, mm, rs, dipole, acc);
