@@ -130,7 +130,7 @@ def adjoint_section(ctx, T):
         prof_ms = ctx.last_device_ms()
         kstats = {}
         for kn in ("ka_bs_targets", "ka_bs_sources", "ka_local", "ka_gather", "ka_bs_finish",
-                   "k_direct_fp64", "k_direct_mixed", "k_geometry", "k_epilogue"):
+                   "k_direct_fp64_small", "k_direct_mixed_small", "k_geometry", "k_epilogue"):
             nl, ms = ctx.kernel_stats(kn)
             if nl:
                 kstats[kn] = {"launches": nl, "total_ms": round(ms, 2)}

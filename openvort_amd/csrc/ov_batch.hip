@@ -301,6 +301,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A) {
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   __shared__ FarLds lds_far[MIXED ? 4 : 1];
+  __shared__ NearLds lds_near[1];   // not staged: a unit walks many source tiles
   const int total = A.ucounts[0];
   for (;;) {
     int u = 0;
@@ -333,8 +334,8 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A) {
     L.adjB = L.valid ? A.adj[2 * L.s + 1] / TILE : -1;
     const TgtTile tt = A.tgt[T];
     double acc[3] = {0, 0, 0};
-    direct_items<MIXED>(A.src, A.far, A.near, A.img + t.img_base, t.n_img, t.rec_base / TILE, tt, L,
-                        w0, w1, A.rnear2, &lds_far[MIXED ? wid : 0], acc);
+    direct_items<MIXED, false>(A.src, A.far, A.near, A.img + t.img_base, t.n_img, t.rec_base / TILE,
+                               tt, L, w0, w1, A.rnear2, &lds_far[MIXED ? wid : 0], &lds_near[0], acc);
     double* o = A.upart + ((long)u * TILE + lane) * 3;
     o[0] = acc[0] * t.kappa4pi;
     o[1] = acc[1] * t.kappa4pi;

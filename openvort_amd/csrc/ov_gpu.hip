@@ -993,8 +993,13 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
     A.partial = ctx->partial;
     A.n_chunks = (int)chunks;
     const dim3 grid((unsigned)((n_tgt + 3) / 4), (unsigned)chunks);
-    if (ctx->mixed)
+    const bool small = n_tgt <= SMALL_TANGLE_TILES;
+    if (ctx->mixed && small)
+      rc = launch(ctx, "k_direct_mixed_small", k_direct_mixed_small, grid, dim3(256), A);
+    else if (ctx->mixed)
       rc = launch(ctx, "k_direct_mixed", k_direct_mixed, grid, dim3(256), A);
+    else if (small)
+      rc = launch(ctx, "k_direct_fp64_small", k_direct_fp64_small, grid, dim3(256), A);
     else
       rc = launch(ctx, "k_direct_fp64", k_direct_fp64, grid, dim3(256), A);
     if (rc) return rc;

@@ -529,6 +529,24 @@ __device__ __forceinline__ Pair64 pair64(const double Rp[3], double l2p, double 
   return o;
 }
 
+// A source tile's FP64 records staged in LDS, one copy per wavefront, read
+// back by broadcast LDS reads (the small-tangle kernels): in small tangles
+// every tile is near, each wavefront's work is a few tiles, and reading the
+// records one by one through the scalar cache leaves the pass waiting on
+// memory.  lane e writes record e;
+// only this wavefront reads or writes the copy, and its LDS operations
+// complete in order.
+struct NearLds {
+  NearRec rec[TILE];
+};
+__device__ __forceinline__ void stage_near(const NearRec* __restrict__ near, int base, int lane,
+                                           NearLds* lds) {
+  lds->rec[lane] = near[base + lane];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // FP64 pass over one source tile for the image target rs of lane target s.
 // Zero rules (l, l' < 1e-8; |cos - 1| < 1e-8) are decided on the estimate,
 // which carries ~1e-15 error: records where some lane's estimate lies within
@@ -536,9 +554,9 @@ __device__ __forceinline__ Pair64 pair64(const double Rp[3], double l2p, double 
 // in a wave-uniform mask (and, per lane, the band flag and the decision
 // taken) and re-decided after the pass with the reference's own expression,
 // correcting the sum where the decisions differ.
-__device__ __forceinline__ void near_tile(const NearRec* __restrict__ rec, const SrcTile& st,
-                                          int base, const double rs[3], int s, bool primary,
-                                          double acc[3]) {
+__device__ __forceinline__ void near_tile(const NearRec* __restrict__ lrec, const NearRec* __restrict__ rec,
+                                          const SrcTile& st, int base, const double rs[3], int s,
+                                          bool primary, double acc[3]) {
   const double R0[3] = {st.prev[0] - rs[0], st.prev[1] - rs[1], st.prev[2] - rs[2]};
   double Rp[3] = {R0[0], R0[1], R0[2]};
   double l2p = Rp[0] * Rp[0] + Rp[1] * Rp[1] + Rp[2] * Rp[2];
@@ -546,7 +564,7 @@ __device__ __forceinline__ void near_tile(const NearRec* __restrict__ rec, const
   uint64_t bandmask = 0, lane_band = 0, lane_zero = 0;
 #pragma unroll 2
   for (int e = 0; e < TILE; ++e) {
-    const NearRec q = rec[base + e];
+    const NearRec q = lrec[e];
     Pair64 o = pair64(Rp, l2p, up, q.p, rs);
     const bool skip = q.seg < 0 || (primary && (q.seg == s || q.seg_fwd == s));
     const bool zero = l2p < 1e-16 || o.l2 < 1e-16 || o.nd1 < 1e-8;
@@ -913,19 +931,21 @@ struct LaneTarget {
 // w / n_img, image w % n_img), walked with an incremental (tile, image)
 // counter; the mirror axis is wave-uniform, so it selects by scalar branches.  MIXED: FP32 pass for source tiles beyond the
 // near radius; otherwise all FP64.  Adds the raw sums (without kappa/4pi).
-template <bool MIXED>
+template <bool MIXED, bool STAGE_NEAR>
 __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
                                              const FarRec* __restrict__ far,
                                              const NearRec* __restrict__ near,
                                              const ImgDev* __restrict__ img, int n_img, int J_base,
                                              const TgtTile& tt, const LaneTarget& L, long w0,
-                                             long w1, double rnear2, FarLds* lds, double acc[3]) {
+                                             long w1, double rnear2, FarLds* lds, NearLds* nlds,
+                                             double acc[3]) {
   const int lane = threadIdx.x & 63;
   // the band threshold as a VGPR operand (a VOP3 compare cannot take a
   // literal, and an SGPR operand would keep it from dual issue)
   float band = SMALL_BAND;
   asm volatile("" : "+v"(band));
   int staged = -1;   // source tile held in lds
+  int staged_near = -1;   // source tile held in nlds
   int Jl = (int)(w0 / n_img);
   int I = (int)(w0 - (long)Jl * n_img);
   for (long w = w0; w < w1; ++w, ++I) {
@@ -978,7 +998,15 @@ __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
       } else
         far_tile(far, near, sj, J * TILE, off, rs, v);
     } else {
-      near_tile(near, sj, J * TILE, rs, L.s, im.primary != 0, v);
+      if (STAGE_NEAR) {
+        if (J != staged_near) {
+          stage_near(near, J * TILE, lane, nlds);
+          staged_near = J;
+        }
+        near_tile(nlds->rec, near, sj, J * TILE, rs, L.s, im.primary != 0, v);
+      } else {
+        near_tile(near + J * TILE, near, sj, J * TILE, rs, L.s, im.primary != 0, v);
+      }
     }
     if (__builtin_expect(im.axis >= 0, 0)) {
 #pragma unroll
@@ -994,7 +1022,7 @@ __device__ __forceinline__ void direct_items(const SrcTile* __restrict__ src,
 // Grid: blockIdx.x = group of 4 target tiles (one wavefront each),
 // blockIdx.y = chunk of the (source tile, image) work list.  Each wave writes
 // its chunk's sum to partial[chunk]; k_reduce adds the chunks in order.
-template <bool MIXED>
+template <bool MIXED, bool STAGE_NEAR>
 __device__ __forceinline__ void direct_body(const DirectArgs& A) {
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1013,8 +1041,9 @@ __device__ __forceinline__ void direct_body(const DirectArgs& A) {
   const TgtTile tt = A.tgt[T];
   double acc[3] = {0, 0, 0};
   __shared__ FarLds lds[MIXED ? 4 : 1];
-  direct_items<MIXED>(A.src, A.far, A.near, A.img, A.n_img, 0, tt, L, w0, w1, A.rnear2,
-                      &lds[MIXED ? wid : 0], acc);
+  __shared__ NearLds nlds[STAGE_NEAR ? 4 : 1];
+  direct_items<MIXED, STAGE_NEAR>(A.src, A.far, A.near, A.img, A.n_img, 0, tt, L, w0, w1, A.rnear2,
+                                  &lds[MIXED ? wid : 0], &nlds[STAGE_NEAR ? wid : 0], acc);
   if (L.valid) {
     double* out = A.partial + (long)chunk * 3 * A.cap;
     out[L.s] = acc[0] * A.kappa4pi;
@@ -1027,8 +1056,16 @@ __device__ __forceinline__ void direct_body(const DirectArgs& A) {
 // its registers (spills stay in the rare band/correction paths) and the extra
 // wave per SIMD hides the scalar-load latency of the record groups (6 % faster
 // than the unconstrained 148-VGPR build at 200k points).
-__global__ void __launch_bounds__(256, 4) k_direct_mixed(DirectArgs A) { direct_body<true>(A); }
-__global__ void __launch_bounds__(256) k_direct_fp64(DirectArgs A) { direct_body<false>(A); }
+__global__ void __launch_bounds__(256, 4) k_direct_mixed(DirectArgs A) { direct_body<true, false>(A); }
+__global__ void __launch_bounds__(256) k_direct_fp64(DirectArgs A) { direct_body<false, false>(A); }
+// Small tangles (at most SMALL_TANGLE_TILES target tiles): each wave's work
+// is one or a few source tiles, and staging their FP64 records in LDS takes
+// the scalar-load latency out of the near pass (4096-point ring: 68 -> 57 us
+// FP64, 61 -> 54 us mixed); at large sizes, where a wave walks thousands of
+// tiles, it does not pay.
+constexpr int SMALL_TANGLE_TILES = 128;
+__global__ void __launch_bounds__(256, 4) k_direct_mixed_small(DirectArgs A) { direct_body<true, true>(A); }
+__global__ void __launch_bounds__(256) k_direct_fp64_small(DirectArgs A) { direct_body<false, true>(A); }
 
 // Grid (points / 256, 3): one component per blockIdx.y, chunks added in order.
 __global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ partial, int n_chunks,
