@@ -848,7 +848,7 @@ int build_tree(ov_ctx* ctx, const double* X) {
   if ((rc = launch(ctx, "k_oct_nodes", k_oct_nodes, dim3((unsigned)((ncell + 256) / 256)), dim3(256),
                    (const OctCell*)ctx->t_cells, ncell, (const double*)ctx->t_mom,
                    (const int*)ctx->t_size, (const int*)ctx->t_pre, ctx->t_nodes, ctx->t_nmom,
-                   ctx->op.theta)))
+                   ctx->op.theta, ctx->mixed ? 1 : 0)))
     return rc;
   return OV_OK;
 }

@@ -49,6 +49,13 @@ struct OctMom {
   double gam[3];
   double M[9];
 };
+// The same moments rounded to FP32 (the FP32-multipole traversal), written
+// over the FP64 array's storage (48 of its 96 bytes per node) when the tree is
+// built for that traversal.
+struct alignas(16) OctMom32 {
+  float gam[3];
+  float M[9];
+};
 
 struct LeafSeg {                   // segment of a sorted point
   double a[3], b[3];
@@ -482,7 +489,8 @@ __global__ void __launch_bounds__(256) k_oct_nodes(const OctCell* __restrict__ c
                                                    const int* __restrict__ size,
                                                    const int* __restrict__ pre,
                                                    OctNode* __restrict__ nodes,
-                                                   OctMom* __restrict__ nmom, double theta) {
+                                                   OctMom* __restrict__ nmom, double theta,
+                                                   int fp32_moments) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c > ncell) return;
   OctNode nd;
@@ -511,7 +519,14 @@ __global__ void __launch_bounds__(256) k_oct_nodes(const OctCell* __restrict__ c
   // theta = 0 gives +inf: never accepted, as lmax / d < 0 never holds
   nd.q = (float)(nd.lmax * nd.lmax / (theta * theta));
   nodes[pre[c]] = nd;
-  nmom[pre[c]] = mm;
+  if (fp32_moments) {
+    OctMom32 m32;
+    for (int a = 0; a < 3; ++a) m32.gam[a] = (float)mm.gam[a];
+    for (int a = 0; a < 9; ++a) m32.M[a] = (float)mm.M[a];
+    reinterpret_cast<OctMom32*>(nmom)[pre[c]] = m32;
+  } else {
+    nmom[pre[c]] = mm;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -596,16 +611,15 @@ __device__ inline void oct_multipole(const double com[3], const OctMom& nd, cons
 
 // FP32 form of oct_multipole: R = r - centre in FP64, everything after in
 // FP32 (the node's moments rounded once).
-__device__ inline void oct_multipole32(const double com[3], const OctMom& nd, const double r[3],
+__device__ inline void oct_multipole32(const double com[3], const OctMom32& nd, const double r[3],
                                        bool dipole, double acc[3]) {
   const float R[3] = {(float)(r[0] - com[0]), (float)(r[1] - com[1]), (float)(r[2] - com[2])};
   const float R2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
   const float ir = __builtin_amdgcn_rsqf(R2);
   const float i3 = ir * ir * ir;
   const float i5 = i3 * ir * ir;
-  float M[9], G[3];
-  for (int a = 0; a < 9; ++a) M[a] = (float)nd.M[a];
-  for (int a = 0; a < 3; ++a) G[a] = (float)nd.gam[a];
+  const float* M = nd.M;
+  const float* G = nd.gam;
   float v[3] = {(G[1] * R[2] - G[2] * R[1]) * i3, (G[2] * R[0] - G[0] * R[2]) * i3,
                 (G[0] * R[1] - G[1] * R[0]) * i3};
   float u[3];
@@ -762,7 +776,8 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
   const int skip_id = (valid && im.primary) ? s : -1;
   const bool dipole = A.op.dipole != 0;
   // 0 in every lane, but a per-lane value to the compiler
-  const float lane_zero = (float)(lane >> 6);
+  const int lane_zero_i = lane >> 6;
+  const float lane_zero = (float)lane_zero_i;
   const OctFast F = oct_fast_consts(A.op.box, lane_zero);
   double acc[3] = {0, 0, 0};
   unsigned leaf_pairs = 0, mps = 0, visits = 0;
@@ -798,11 +813,16 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
           dec = oct_accept(dv, nd.lmax, A.op.theta) ? 1 : 0;
         }
         if (dec) {
-          const OctMom mm = A.nmom[node];
-          if (MIXED)
+          if (MIXED) {
+            // a vector load of the FP32 moments (the index looks per-lane to
+            // the compiler): they arrive in VGPRs, so the multipole's FP32
+            // instructions dual-issue, where SGPR operands would not
+            const OctMom32 mm = reinterpret_cast<const OctMom32*>(A.nmom)[node + lane_zero_i];
             oct_multipole32(nd.com, mm, rs, dipole, acc);
-          else
+          } else {
+            const OctMom mm = A.nmom[node];
             oct_multipole(nd.com, mm, rs, dipole, acc);
+          }
           ++mps;
           next = after;
         } else {
