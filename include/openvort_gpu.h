@@ -237,7 +237,9 @@ void ov_ctx_batch_tuning(const ov_ctx *ctx, int *unit, int *blocks, double *tune
  * tangle's positions, FP64, direct sum (use_BH must be 0; precision selects
  * the forward sum whose values enter the mutual-friction terms).  Both arrays
  * have the tangle's capacity N; EMPTY entries of grad_vnodes are set to 0.
- * Flags: OV_PROFILE.  No counterpart in the reference (it has no gradients). */
+ * Every imposed flow of the table is differentiated (its spatial Jacobian at
+ * time t).  Flags: OV_PROFILE.  No counterpart in the reference (it has no
+ * gradients). */
 int ov_gpu_velocity_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t,
                         const ov_params *params, const ov_vec3d *cot_vels,
                         ov_vec3d *grad_vnodes, int flags);
@@ -250,8 +252,9 @@ int ov_gpu_velocity_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t,
  * segment being differentiated, every step's stage data (8 N x 24 bytes per
  * step).  OV_ENFORCE_BOUNDARIES wraps as the forward did (the wrap's
  * derivative is the identity); OV_USE_CALLER_K1 is rejected.  EMPTY points
- * do not move, so their entries of grad_initial are cot_final's.  The tangle
- * is not modified.  No counterpart in the reference. */
+ * do not move, so their entries of grad_initial are cot_final's.  Stage k
+ * of the step from t sees the flows at the forward's stage times.  The
+ * tangle is not modified.  No counterpart in the reference. */
 int ov_gpu_rk4_run_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t0, double dt,
                        int nsteps, const ov_params *params, int flags, int checkpoint_every,
                        const ov_vec3d *cot_final, ov_vec3d *grad_initial);

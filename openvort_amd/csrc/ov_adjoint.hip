@@ -8,6 +8,9 @@
 //     — differentiated per point with dual numbers (value + 3 seeded
 //     directions; 6 seedings: the five stencil points and u_k); the five
 //     transposed blocks are gathered back to the points they came from;
+//   * the imposed flows vs, vn (external_velocity.c:176-512) at p_k and time
+//     t — evaluated once per point on duals seeded along x, y, z, their 3x3
+//     Jacobians entering the seeding of p_k only;
 //   * the segment law (tangle.c:486-514) summed over segments and images:
 //     for one pair, with R = a - r, R' = b - r, l = |R|, l' = |R'|,
 //     d = R.R', q = l l' + d, g = l l' q, f = (l + l') / g, c = R x R',
@@ -18,11 +21,12 @@
 //     f_d = -f l l' / g.  The target gets -(dB/dR + dB/dR'), the segment's
 //     start dB/dR and its (minimum-image) end dB/dR'.  Pairs the reference
 //     zeroes (l or l' < 1e-8, |R.R'/(l l') - 1| < 1e-8) contribute nothing.
-// Branch points (the minimum image, the hyperfriction switch) are
-// differentiated on the side the values select.
+// Branch points (the minimum image, the hyperfriction switch, the flows'
+// cut-offs) are differentiated on the side the values select.
 #include <hip/hip_runtime.h>
 
 #include "ov_device.h"
+#include "ov_flow.h"
 
 namespace ov {
 
@@ -54,6 +58,133 @@ __device__ inline D3 dsqrt(const D3& a) {
   return {r, {a.d[0] * h, a.d[1] * h, a.d[2] * h}};
 }
 __device__ inline D3 dlog(const D3& a) { return {log(a.v), {a.d[0] / a.v, a.d[1] / a.v, a.d[2] / a.v}}; }
+__device__ inline D3 operator+(const D3& a, double s) { return {a.v + s, {a.d[0], a.d[1], a.d[2]}}; }
+__device__ inline D3 operator+(double s, const D3& a) { return a + s; }
+__device__ inline D3 operator-(const D3& a, double s) { return {a.v - s, {a.d[0], a.d[1], a.d[2]}}; }
+__device__ inline D3 operator-(double s, const D3& a) { return {s - a.v, {-a.d[0], -a.d[1], -a.d[2]}}; }
+__device__ inline D3 operator/(const D3& a, double s) { return {a.v / s, {a.d[0] / s, a.d[1] / s, a.d[2] / s}}; }
+__device__ inline D3 operator/(double s, const D3& a) { return dc(s) / a; }
+__device__ inline D3 dscale(const D3& a, double f, double df) {   // g(a) with g = f, g' = df
+  return {f, {a.d[0] * df, a.d[1] * df, a.d[2] * df}};
+}
+__device__ inline D3 dexp(const D3& a) {
+  const double e = exp(a.v);
+  return dscale(a, e, e);
+}
+__device__ inline D3 dcos(const D3& a) { return dscale(a, cos(a.v), -sin(a.v)); }
+__device__ inline D3 dsin(const D3& a) { return dscale(a, sin(a.v), cos(a.v)); }
+
+// The imposed flow of descriptor f at the dual position r and time t: the
+// formulas of flow_eval (ov_flow.h, external_velocity.c:176-512) on dual
+// numbers, so the point velocity's VJP carries the flows' spatial Jacobians.
+// Branches (cut-offs) are taken on the values.
+__device__ inline void dflow_eval(const ov_flow& f, const D3 r[3], double t, D3 out[3]) {
+  const D3 x = r[0], y = r[1], z = r[2];
+  const D3 zero = dc(0);
+  auto spherical = [&](D3 o[3]) {
+    const double strength = f.a[0], cutoff = f.a[1];
+    const D3 d = dsqrt(x * x + y * y + z * z);
+    const D3 attn = (cutoff / d) * (cutoff / d);
+    if (d.v < 1e-8 || attn.v > 100) {
+      o[0] = o[1] = o[2] = zero;
+      return;
+    }
+    const D3 m = strength / (4 * FLOW_PI * d * d) * dexp(-attn);
+    for (int i = 0; i < 3; ++i) o[i] = (r[i] / d) * m;
+  };
+  switch (f.type) {
+    case OV_FLOW_SIMPLE:
+      for (int i = 0; i < 3; ++i) out[i] = dc(f.v[i]);
+      return;
+    case OV_FLOW_SPHERICAL:
+      spherical(out);
+      return;
+    case OV_FLOW_SPHERICAL_AND_SIMPLE:
+      spherical(out);
+      for (int i = 0; i < 3; ++i) out[i] = out[i] + f.v[i];
+      return;
+    case OV_FLOW_CYLINDRICAL: {
+      const double strength = f.a[0], cutoff = f.a[1];
+      const D3 d = dsqrt(x * x + y * y);
+      const D3 attn = (cutoff / d) * (cutoff / d);
+      if (d.v < 1e-8 || attn.v > 100) {
+        out[0] = out[1] = out[2] = zero;
+        return;
+      }
+      const D3 m = strength / (2 * FLOW_PI * d) * dexp(-attn);
+      out[0] = (x / d) * m;
+      out[1] = (y / d) * m;
+      out[2] = zero;
+      return;
+    }
+    case OV_FLOW_OSCILLATING: {
+      const double strength = f.a[0], freq = f.a[1];
+      const D3 kr = f.w[0] * x + f.w[1] * y + f.w[2] * z;
+      const D3 m = dcos(2 * FLOW_PI * kr) * (strength * sin(2 * FLOW_PI * freq * t));
+      for (int i = 0; i < 3; ++i) out[i] = f.q[i] * m;
+      return;
+    }
+    case OV_FLOW_OSCILLATING_SLIT: {
+      const double freq = f.a[1], cutoff = f.a[2];
+      const D3 rr = dsqrt(x * x + z * z);
+      const D3 m = dexp(-cutoff * cutoff / rr / rr) * cos(2 * FLOW_PI * freq * t);
+      out[0] = (x / rr) * m;
+      out[1] = zero;
+      out[2] = (z / rr) * m;
+      return;
+    }
+    case OV_FLOW_COSCOS: {
+      const double k = f.a[0], v0 = f.a[1], l = f.a[2];
+      out[0] = out[1] = zero;
+      out[2] = v0 * (1 + dcos(k * x) * dcos(k * y) * dexp(-z / l));
+      return;
+    }
+    case OV_FLOW_COSCOS_DIVFREE: {
+      const double k = f.a[0], v0 = f.a[1], l = f.a[2];
+      const D3 e = dexp(-z / l);
+      out[0] = v0 / 2 / l / k * dsin(k * x) * dcos(k * y) * e;
+      out[1] = v0 / 2 / l / k * dcos(k * x) * dsin(k * y) * e;
+      out[2] = v0 * (1 + dcos(k * x) * dcos(k * y) * e);
+      return;
+    }
+    case OV_FLOW_COS_DIVFREE: {
+      const double k = f.a[0], v0 = f.a[1], l = f.a[2];
+      const D3 e = dexp(-z / l);
+      out[0] = v0 / l / k * dsin(k * x) * e;
+      out[1] = zero;
+      out[2] = v0 * (1 + dcos(k * x) * e);
+      return;
+    }
+    case OV_FLOW_SIMPLE_SHEAR:
+      out[0] = out[1] = zero;
+      out[2] = f.a[1] + f.a[0] * x;
+      return;
+    case OV_FLOW_ROTOSC:
+    case OV_FLOW_ROTOSC_BOUNDARY: {
+      const double m = f.a[1] * cos(2 * FLOW_PI * f.a[0] * t);
+      out[0] = -y * m;
+      out[1] = x * m;
+      out[2] = zero;
+      return;
+    }
+    case OV_FLOW_DG:
+      out[0] = -y * f.a[0];
+      out[1] = x * f.a[0];
+      out[2] = dc(f.a[1]);
+      return;
+    case OV_FLOW_CONSTANT_ACCELERATION:
+      out[0] = dc(f.a[0] * t);
+      out[1] = out[2] = zero;
+      return;
+    case OV_FLOW_PARABOLIC:
+      out[0] = f.a[1] * z * (f.a[0] - z);
+      out[1] = out[2] = zero;
+      return;
+    default:
+      out[0] = out[1] = out[2] = zero;
+      return;
+  }
+}
 
 __device__ inline void dcross(const D3 a[3], const D3 b[3], D3 o[3]) {
   const D3 x = a[1] * b[2] - a[2] * b[1];
@@ -90,7 +221,7 @@ __device__ inline D3 dchord(const D3 s[3], const D3 sp[3], const BoxDev& b) {
 // epilogue_point): p[0..4] = positions of k-2, k-1, k, k+1, k+2, u = the
 // Biot-Savart sum at k.
 __device__ void local_velocity(const D3 p[5][3], const D3 u[3], const BoxDev& box, const PhysDev& ph,
-                               D3 vel[3]) {
+                               const D3 ext[3], const D3 vn[3], D3 vel[3]) {
   const D3 dk = dchord(p[2], p[3], box), df = dchord(p[3], p[4], box);
   const D3 dr = dchord(p[1], p[2], box), drr = dchord(p[0], p[1], box);
   const D3 x[4] = {dk + df, dk, -dr, -(dr + drr)};
@@ -119,9 +250,7 @@ __device__ void local_velocity(const D3 p[5][3], const D3 u[3], const BoxDev& bo
   dcross(t, nm, lia);
   for (int i = 0; i < 3; ++i) lia[i] = lia[i] * beta;
   D3 vs[3];
-  // imposed flows: constant ("no flow", "simple") only; the entry points
-  // reject the others
-  for (int i = 0; i < 3; ++i) vs[i] = lia[i] + dc(ph.vs.type == OV_FLOW_SIMPLE ? ph.vs.v[i] : 0.0) + u[i];
+  for (int i = 0; i < 3; ++i) vs[i] = lia[i] + ext[i] + u[i];
   for (int i = 0; i < 3; ++i) vel[i] = vs[i];
   if (ph.hyperfriction) {
     const double spp = sqrt(nm[0].v * nm[0].v + nm[1].v * nm[1].v + nm[2].v * nm[2].v);
@@ -133,7 +262,7 @@ __device__ void local_velocity(const D3 p[5][3], const D3 u[3], const BoxDev& bo
   }
   if (ph.use_mf) {
     D3 dv[3], a1[3], a2[3];
-    for (int i = 0; i < 3; ++i) dv[i] = dc(ph.vn.type == OV_FLOW_SIMPLE ? ph.vn.v[i] : 0.0) - vs[i];
+    for (int i = 0; i < 3; ++i) dv[i] = vn[i] - vs[i];
     dcross(t, dv, a1);
     dcross(t, a1, a2);
     for (int i = 0; i < 3; ++i) vel[i] = vel[i] + a1[i] * ph.alpha - a2[i] * ph.alpha_p;
@@ -147,8 +276,8 @@ __global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
                                                 const int* __restrict__ rev,
                                                 const double* __restrict__ bs, int have_bs,
                                                 const double* __restrict__ w, int n, long cap,
-                                                BoxDev box, PhysDev ph, double* __restrict__ gnb,
-                                                double* __restrict__ gu) {
+                                                BoxDev box, PhysDev ph, double t,
+                                                double* __restrict__ gnb, double* __restrict__ gu) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int ids[5];
@@ -160,6 +289,22 @@ __global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
   double wk[3], uk[3] = {0, 0, 0};
   load3(w, cap, k, wk);
   if (have_bs) load3(bs, cap, k, uk);
+  // The imposed flows depend on the point's own position only: evaluate them
+  // and their 3x3 Jacobians once, seeded along x, y, z of point k, and hand
+  // the derivative part only to the seeding of point k (slot 2).
+  D3 fs[3], fn[3];
+  {
+    D3 r[3];
+    for (int i = 0; i < 3; ++i) {
+      r[i] = dc(P[i * cap + k]);
+      r[i].d[i] = 1.0;
+    }
+    dflow_eval(ph.vs, r, t, fs);
+    if (ph.use_mf)
+      dflow_eval(ph.vn, r, t, fn);
+    else
+      fn[0] = fn[1] = fn[2] = dc(0);
+  }
   for (int seed = 0; seed < 6; ++seed) {
     D3 p[5][3], u[3];
     for (int j = 0; j < 5; ++j)
@@ -171,8 +316,13 @@ __global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
       u[i] = dc(uk[i]);
       if (seed == 5) u[i].d[i] = 1.0;
     }
+    D3 ext[3], vn[3];
+    for (int i = 0; i < 3; ++i) {
+      ext[i] = seed == 2 ? fs[i] : dc(fs[i].v);
+      vn[i] = seed == 2 ? fn[i] : dc(fn[i].v);
+    }
     D3 vel[3];
-    local_velocity(p, u, box, ph, vel);
+    local_velocity(p, u, box, ph, ext, vn, vel);
     double g[3];
     for (int c = 0; c < 3; ++c) g[c] = vel[0].d[c] * wk[0] + vel[1].d[c] * wk[1] + vel[2].d[c] * wk[2];
     if (seed < 5) {

@@ -1293,15 +1293,17 @@ int adj_buffers(ov_ctx* ctx) {
   return OV_OK;
 }
 
-// G = (dvels/dX)^T W at positions X whose Biot-Savart sums are `bs`.
-int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const double* W, double* G) {
+// G = (dvels/dX)^T W at positions X whose Biot-Savart sums are `bs`, time t
+// (the imposed flows').
+int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const double* W, double t,
+                     double* G) {
   const int n = ctx->n;
   const long cap = ctx->cap;
   const int have_bs = ctx->lia_only ? 0 : 1;
   int rc;
   if ((rc = launch(ctx, "ka_local", ka_local, dim3((unsigned)((n + 127) / 128)), dim3(128), X,
                    (const int*)ctx->fwd, (const int*)ctx->rev, bs, have_bs, W, n, cap, ctx->box,
-                   ctx->ph, ctx->a_gnb, ctx->a_gu)))
+                   ctx->ph, t, ctx->a_gnb, ctx->a_gu)))
     return rc;
   if ((rc = launch(ctx, "ka_gather", ka_gather, grid1(n), dim3(256), (const double*)ctx->a_gnb,
                    (const int*)ctx->fwd, (const int*)ctx->rev, n, cap, ctx->a_gl)))
@@ -1345,12 +1347,14 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
 }
 
 // On entry G = cotangent of the step's end positions; on return the cotangent
-// of its start positions.  `store` is what rk4_device kept for the step.
-int step_vjp_dev(ov_ctx* ctx, const double* store, double dt, int rk_variant, double* G) {
+// of its start positions.  `store` is what rk4_device kept for the step that
+// starts at time t (stages at t, t + dt/2, t + dt/2, t + dt, as rk4_device).
+int step_vjp_dev(ov_ctx* ctx, const double* store, double t, double dt, int rk_variant, double* G) {
   const int n = ctx->n;
   const long cap = ctx->cap;
   const long ps = 3 * cap;
   const double h[3] = {dt / 2, dt / 2, rk_variant == OV_RK_CLASSIC ? dt : dt / 2};
+  const double ts[3] = {t + dt / 2, t + dt / 2, t + dt};
   const double wk[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   int rc;
   for (int i = 0; i < 4; ++i)
@@ -1366,7 +1370,8 @@ int step_vjp_dev(ov_ctx* ctx, const double* store, double dt, int rk_variant, do
         return rc;
       Xs = ctx->Xs;
     }
-    if ((rc = velocity_vjp_dev(ctx, Xs, store + (5 + st) * ps, ctx->a_gK[st + 1], ctx->a_Gtmp)))
+    if ((rc = velocity_vjp_dev(ctx, Xs, store + (5 + st) * ps, ctx->a_gK[st + 1], st >= 0 ? ts[st] : t,
+                               ctx->a_Gtmp)))
       return rc;
     if ((rc = launch(ctx, "ka_axpy", ka_axpy, grid1(n), dim3(256), (const double*)G, 1.0,
                      (const double*)ctx->a_Gtmp, G, n, cap)))
@@ -1385,9 +1390,6 @@ int adj_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   if (rc) return rc;
   if (p->use_BH)
     return fail(ctx, OV_E_UNSUPPORTED, "the adjoint differentiates the direct sum only (use_BH=0)");
-  for (const ov_flow* f : {&p->vs, &p->vn})
-    if (f->type != OV_FLOW_NONE && f->type != OV_FLOW_SIMPLE)
-      return fail(ctx, OV_E_UNSUPPORTED, "the adjoint takes the 'no flow' and 'simple' flows only");
   if (ctx->n == 0) return OV_OK;
   if ((rc = adj_buffers(ctx))) return rc;
   return upload_topology(ctx);
@@ -2266,7 +2268,7 @@ int ov_gpu_velocity_vjp(ov_ctx* ctx, const ov_tangle_state* t, double time, cons
     return rc;
   if ((rc = mark_begin(ctx))) return rc;
   if ((rc = evaluate(ctx, ctx->X0, true, ctx->T, ctx->Nn, ctx->dxi, ctx->vs, ctx->K[0], time)) ||
-      (rc = velocity_vjp_dev(ctx, ctx->X0, ctx->bs, ctx->a_W, ctx->a_G)) ||
+      (rc = velocity_vjp_dev(ctx, ctx->X0, ctx->bs, ctx->a_W, time, ctx->a_G)) ||
       (rc = launch(ctx, "ka_finite", ka_finite, grid1(ctx->n), dim3(256), (const double*)ctx->a_G,
                    ctx->n, ctx->cap, ctx->flag)))
     return rc;
@@ -2320,14 +2322,16 @@ int ov_gpu_rk4_run_vjp(ov_ctx* ctx, const ov_tangle_state* t, double t0, double 
     }
     return r;
   };
-  auto backward = [&](int count) {
+  auto backward = [&](int s0, int count) {
     int r = OV_OK;
     for (int j = count - 1; j >= 0 && !r; --j)
-      r = step_vjp_dev(ctx, ctx->a_store + (long)j * 8 * ps, dt, p->rk_variant, ctx->a_G);
+      r = step_vjp_dev(ctx, ctx->a_store + (long)j * 8 * ps, t0 + (s0 + j) * dt, dt, p->rk_variant,
+                       ctx->a_G);
     return r;
   };
-  // full segments after the first eager one are graphs (no profiling)
-  const bool graphs = !ctx->profile;
+  // full segments after the first eager one are graphs (no profiling; none
+  // when an imposed flow depends on t, which a captured launch would fix)
+  const bool graphs = !ctx->profile && !ctx->flow_time_dep;
   hipGraphExec_t g_fwd = nullptr, g_seg = nullptr;
   auto cleanup = [&](int r) {
     if (g_fwd) hipGraphExecDestroy(g_fwd);
@@ -2354,12 +2358,12 @@ int ov_gpu_rk4_run_vjp(ov_ctx* ctx, const ov_tangle_state* t, double t0, double 
     if (graphs && sized && count == k) {
       if (!g_seg && (rc = capture_graph(ctx, [&]() {
                        int r = forward(s0, k, true);
-                       return r ? r : backward(k);
+                       return r ? r : backward(s0, k);
                      }, &g_seg)))
         return cleanup(rc);
       if ((rc = launch_graph(ctx, g_seg, 1))) return cleanup(rc);
     } else {
-      if ((rc = forward(s0, count, true)) || (rc = backward(count))) return cleanup(rc);
+      if ((rc = forward(s0, count, true)) || (rc = backward(s0, count))) return cleanup(rc);
       sized = true;
     }
   }
