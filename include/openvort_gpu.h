@@ -182,7 +182,7 @@ void *ov_ctx_stream(ov_ctx *ctx);
 /* update_tangents_normals (tangle.h:120): writes dxi, tangents, normals.
  *
  * Walls (this call, ov_gpu_velocities and ov_gpu_rk4_step / ov_gpu_rk4_run,
- * direct sum or octree): PINNED and PINNED_SLIP points with pin_wall 0..5
+ * direct sum or octree, the topology edits and ov_gpu_run): PINNED and PINNED_SLIP points with pin_wall 0..5
  * may end open chains (forward or reverse -1).  Neighbours past a chain end
  * are the mirror images of the chain in that point's wall (step_node,
  * tangle.c:165-198); PINNED points take the one-sided stencil (tangle.c:
@@ -307,16 +307,23 @@ void ov_topology_default(ov_topology *tp);
  * slot left) it returns OV_E_NOMEM and leaves the tangle unchanged: pass a
  * tangle with spare capacity.  add_point's curvature test uses
  * params->max_curvature_scale / params->dl_max (global_max_curvature_scale,
- * global_dl_max). */
+ * global_dl_max).  With points pinned on walls the refresh is the walls'
+ * geometry (above) and a pinned neighbour is not moved by remove_point;
+ * a removal that leaves two neighbouring points between PINNED points
+ * (where the reference's update_tangent_normal recurses without end) stops
+ * the call with OV_E_INVALID. */
 int ov_gpu_remesh(ov_ctx *ctx, ov_tangle_state *tangle, const ov_params *params,
                   double min_dist, double max_dist);
 
 /* eliminate_small_loops(tangle, tp->small_loop_cutoff) (tangle.c:878-945)
  * with the origin / z-axis / outer eliminations it runs first when enabled
  * (tangle.c:947-1015): every loop of fewer than small_loop_cutoff + 1
- * points, or lying wholly inside (outside) the cutoff, is removed.  Writes
- * links and status.  The reference's use of `recalculate` as scratch marks
- * is not reproduced. */
+ * points, or lying wholly inside (outside) the cutoff, is removed.  An open
+ * chain of L points ending on walls counts L - 1 steps as a loop of L points
+ * does and goes as a whole; the origin / z-axis / outer eliminations walk
+ * closed loops only and return OV_E_UNSUPPORTED when points are pinned.
+ * Writes links and status.  The reference's use of `recalculate` as scratch
+ * marks is not reproduced. */
 int ov_gpu_eliminate_small_loops(ov_ctx *ctx, ov_tangle_state *tangle, const ov_topology *tp);
 
 /* enforce_boundaries (tangle.c:796-819): periodic wrap of every point. */
@@ -326,8 +333,9 @@ int ov_gpu_enforce_boundaries(ov_ctx *ctx, ov_tangle_state *tangle);
  * resident on the device: update_tangle(t); rk4_step(t, dt);
  * update_tangents_normals; remesh(dl_min, dl_max); eliminate_small_loops;
  * enforce_boundaries; t += dt.  The point stream is rebuilt on the device
- * every step (loops in order of their lowest index, as every other entry
- * point orders them), so each step equals ov_gpu_rk4_step followed by the
+ * every step (loops and open chains in order of their lowest index, a chain
+ * from its head, as every other entry point orders them, with the pin codes
+ * of points on walls), so each step equals ov_gpu_rk4_step followed by the
  * three edits above.  Writes vnodes, links, status, next_free, the geometry
  * after the last step's edits, and vels / vs of the last step's k1 (slots
  * created afterwards keep the values the slot held, as in the reference).

@@ -168,7 +168,8 @@ struct ov_ctx {
   int *ix_fwd = nullptr, *ix_rev = nullptr, *ix_status = nullptr;
   int *ix_lab[2] = {nullptr, nullptr}, *ix_jmp[2] = {nullptr, nullptr}, *ix_d[2] = {nullptr, nullptr};
   int *ix_len = nullptr, *ix_off = nullptr, *ix_pos = nullptr, *ix_act = nullptr, *ix_bits = nullptr,
-      *ix_all = nullptr, *ix_state = nullptr;
+      *ix_all = nullptr, *ix_state = nullptr, *ix_pin = nullptr, *ix_chain = nullptr;
+  bool ix_has_pins = false;        // points pinned on walls (open chains, pin codes in ix_pin)
   std::vector<int> ix_status_in;   // caller status on entry (new points get pin_wall NOT_A_FACE)
 };
 
@@ -436,7 +437,8 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p, bool allo
     if (st == OV_PINNED || st == OV_PINNED_SLIP) {
       if (!allow_pins)
         return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are supported by ov_gpu_velocities, "
-                    "ov_gpu_rk4_step/run (direct sum) and ov_gpu_update_tangents_normals only", k);
+                    "ov_gpu_rk4_step/run, ov_gpu_update_tangents_normals, the topology edits and "
+                    "ov_gpu_run only", k);
       if (t->status[k].pin_wall < 0 || t->status[k].pin_wall > 5)
         return fail(ctx, OV_E_INVALID, "pinned point %d has pin_wall %d", k, t->status[k].pin_wall);
       pins = true;
@@ -1464,7 +1466,8 @@ int ix_buffers(ov_ctx* ctx, int N) {
   if ((rc = dalloc(ctx, (void**)&ctx->ix_dxi, v1))) return rc;
   int** i1b[] = {&ctx->ix_fwd, &ctx->ix_rev, &ctx->ix_status, &ctx->ix_lab[0], &ctx->ix_lab[1],
                  &ctx->ix_jmp[0], &ctx->ix_jmp[1], &ctx->ix_d[0], &ctx->ix_d[1], &ctx->ix_len,
-                 &ctx->ix_off, &ctx->ix_pos, &ctx->ix_act, &ctx->ix_bits, &ctx->ix_all};
+                 &ctx->ix_off, &ctx->ix_pos, &ctx->ix_act, &ctx->ix_bits, &ctx->ix_all,
+                 &ctx->ix_pin, &ctx->ix_chain};
   for (int** b : i1b)
     if ((rc = dalloc(ctx, (void**)b, i1))) return rc;
   if ((rc = dalloc(ctx, (void**)&ctx->ix_state, sizeof(int) * 8))) return rc;
@@ -1472,8 +1475,10 @@ int ix_buffers(ov_ctx* ctx, int N) {
   return OV_OK;
 }
 
-// Validate a tangle for the index-space edits (free points on closed loops,
-// consistent links; pinned points are not supported) and set box/physics.
+// Validate a tangle for the index-space edits and set box/physics: free
+// points on closed loops with consistent links, or, with points pinned on
+// walls, the open chains and pins that the velocity path takes (validated
+// by prepare()).
 int ix_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   if (!t) return fail(ctx, OV_E_INVALID, "null tangle");
   ctx->tree_order_device = false;
@@ -1481,22 +1486,28 @@ int ix_prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p) {
   if (N < 0) return fail(ctx, OV_E_INVALID, "negative N");
   if (N > 0 && (!t->vnodes || !t->connections || !t->status))
     return fail(ctx, OV_E_INVALID, "null tangle arrays");
-  for (int k = 0; k < N; ++k) {
+  ov_params q;
+  if (p)
+    q = *p;
+  else
+    ov_params_default(&q);
+  bool pins = false;
+  for (int k = 0; k < N && !pins; ++k)
+    pins = t->status[k].status == OV_PINNED || t->status[k].status == OV_PINNED_SLIP;
+  ctx->ix_has_pins = pins;
+  if (pins) {
+    int rc = prepare(ctx, t, &q, true);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < N && !pins; ++k) {
     const int st = t->status[k].status;
     if (st == OV_EMPTY) continue;
-    if (st == OV_PINNED || st == OV_PINNED_SLIP)
-      return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: the topology edits take free points only", k);
     if (st != OV_FREE) return fail(ctx, OV_E_INVALID, "bad status %d at %d", st, k);
     const int f = t->connections[k].forward, r = t->connections[k].reverse;
     if (f < 0 || f >= N || r < 0 || r >= N || t->status[f].status != OV_FREE ||
         t->status[r].status != OV_FREE || t->connections[f].reverse != k)
       return fail(ctx, OV_E_INVALID, "point %d: links %d/%d inconsistent", k, f, r);
   }
-  ov_params q;
-  if (p)
-    q = *p;
-  else
-    ov_params_default(&q);
   ImgDev il[MAX_IMAGES];
   int nil = 0;
   int rc = setup_box_physics(ctx, &t->box, &t->bimg, &q, il, &nil);
@@ -1546,13 +1557,16 @@ int ix_upload(ov_ctx* ctx, const ov_tangle_state* t, bool geom, bool vel) {
   if (vel) {
     if ((rc = ix_up3(ctx, t->vels, ctx->ix_vels)) || (rc = ix_up3(ctx, t->vs, ctx->ix_vs))) return rc;
   }
-  std::vector<int> f(N > 0 ? N : 1), r(N > 0 ? N : 1), st(N > 0 ? N : 1);
+  std::vector<int> f(N > 0 ? N : 1), r(N > 0 ? N : 1), st(N > 0 ? N : 1), pc(N > 0 ? N : 1, -1);
   std::vector<double> dx(N > 0 ? N : 1, 0.0);
   ctx->ix_status_in.assign(N, OV_EMPTY);
   for (int k = 0; k < N; ++k) {
     f[k] = t->connections[k].forward;
     r[k] = t->connections[k].reverse;
-    st[k] = t->status[k].status == OV_EMPTY ? OV_EMPTY : OV_FREE;
+    const int s0 = t->status[k].status;
+    st[k] = s0 == OV_EMPTY ? OV_EMPTY : ctx->ix_has_pins ? s0 : OV_FREE;
+    if (s0 == OV_PINNED || s0 == OV_PINNED_SLIP)
+      pc[k] = t->status[k].pin_wall | (s0 == OV_PINNED_SLIP ? 8 : 0) | (f[k] > 0 ? 16 : 0);
     ctx->ix_status_in[k] = st[k];
     if (geom) dx[k] = t->dxi[k];
   }
@@ -1560,6 +1574,8 @@ int ix_upload(ov_ctx* ctx, const ov_tangle_state* t, bool geom, bool vel) {
     HIPCHK(hipMemcpyAsync(ctx->ix_fwd, f.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->ix_rev, r.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->ix_status, st.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->ix_has_pins)
+      HIPCHK(hipMemcpyAsync(ctx->ix_pin, pc.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     if (geom)
       HIPCHK(hipMemcpyAsync(ctx->ix_dxi, dx.data(), sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1599,7 +1615,8 @@ int ix_download(ov_ctx* ctx, ov_tangle_state* t, bool geom, bool vel) {
   for (int k = 0; k < N; ++k) {
     t->connections[k].forward = f[k];
     t->connections[k].reverse = r[k];
-    if (st[k] != OV_EMPTY && ctx->ix_status_in[k] == OV_EMPTY) t->status[k].pin_wall = -1;
+    // a slot that holds a point add_point made (FREE now, not FREE on entry)
+    if (st[k] == OV_FREE && ctx->ix_status_in[k] != OV_FREE) t->status[k].pin_wall = -1;
     t->status[k].status = st[k];
   }
   return OV_OK;
@@ -1614,6 +1631,7 @@ TopoArrays ix_arrays(ov_ctx* ctx) {
   A.fwd = ctx->ix_fwd;
   A.rev = ctx->ix_rev;
   A.status = ctx->ix_status;
+  A.pin = ctx->ix_has_pins ? ctx->ix_pin : nullptr;
   A.cap = ctx->ix_alloc;
   A.N = ctx->ix_N;
   return A;
@@ -1642,6 +1660,10 @@ int ix_remesh(ov_ctx* ctx, double min_dist, double max_dist, double max_curv, do
     return fail(ctx, OV_E_NOMEM,
                 "remesh: no EMPTY slot left in a tangle of %d points (the reference would "
                 "expand_tangle to %d): pass a tangle with more capacity", ctx->ix_N, 2 * ctx->ix_N);
+  if (st[2] == 2)
+    return fail(ctx, OV_E_INVALID,
+                "remesh left two neighbouring points between PINNED points (the reference's "
+                "update_tangent_normal recurses without end there)");
   return OV_OK;
 }
 
@@ -1665,9 +1687,34 @@ int ix_label_loops(ov_ctx* ctx) {
   if (rounds & 1)
     HIPCHK(hipMemcpyAsync(ctx->ix_lab[0], ctx->ix_lab[1], sizeof(int) * N, hipMemcpyDeviceToDevice,
                           ctx->stream));
-  if ((rc = launch(ctx, "k_rank_init", k_rank_init, grid1(N), dim3(256), (const int*)ctx->ix_rev,
-                   (const int*)ctx->ix_lab[0], N, ctx->ix_d[0], ctx->ix_jmp[0])))
+  const bool ch = ctx->ix_has_pins;
+  if (ch) {
+    // open chains: the lowest index over the reverse links too, the chain
+    // flags (count in ix_state[4]) and ranks from the chain heads
+    if ((rc = launch(ctx, "k_jmp_rev", k_jmp_rev, grid1(N), dim3(256), (const int*)ctx->ix_rev,
+                     (const int*)ctx->ix_lab[0], N, ctx->ix_jmp[0])))
+      return rc;
+    for (int q = 0; q < rounds; ++q) {
+      const int a = q & 1, b = a ^ 1;
+      if ((rc = launch(ctx, "k_loop_min", k_loop_min, grid1(N), dim3(256), (const int*)ctx->ix_lab[a],
+                       (const int*)ctx->ix_jmp[a], N, ctx->ix_lab[b], ctx->ix_jmp[b])))
+        return rc;
+    }
+    if (rounds & 1)
+      HIPCHK(hipMemcpyAsync(ctx->ix_lab[0], ctx->ix_lab[1], sizeof(int) * N, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    if ((rc = launch(ctx, "k_fill_int", k_fill_int, grid1(N), dim3(256), ctx->ix_chain, N, 0)) ||
+        (rc = launch(ctx, "k_fill_int", k_fill_int, dim3(1), dim3(256), ctx->ix_state + 4, 1, 0)) ||
+        (rc = launch(ctx, "k_chain_flag", k_chain_flag, grid1(N), dim3(256), (const int*)ctx->ix_rev,
+                     (const int*)ctx->ix_lab[0], N, ctx->ix_chain, ctx->ix_state + 4)) ||
+        (rc = launch(ctx, "k_rank_init_ch", k_rank_init_ch, grid1(N), dim3(256), (const int*)ctx->ix_rev,
+                     (const int*)ctx->ix_lab[0], (const int*)ctx->ix_chain, N, ctx->ix_d[0],
+                     ctx->ix_jmp[0])))
+      return rc;
+  } else if ((rc = launch(ctx, "k_rank_init", k_rank_init, grid1(N), dim3(256), (const int*)ctx->ix_rev,
+                          (const int*)ctx->ix_lab[0], N, ctx->ix_d[0], ctx->ix_jmp[0]))) {
     return rc;
+  }
   for (int q = 0; q < rounds; ++q) {
     const int a = q & 1, b = a ^ 1;
     if ((rc = launch(ctx, "k_rank_step", k_rank_step, grid1(N), dim3(256), (const int*)ctx->ix_d[a],
@@ -1677,14 +1724,31 @@ int ix_label_loops(ov_ctx* ctx) {
   if (rounds & 1)
     HIPCHK(hipMemcpyAsync(ctx->ix_d[0], ctx->ix_d[1], sizeof(int) * N, hipMemcpyDeviceToDevice,
                           ctx->stream));
+  if (ch)
+    return (rc = launch(ctx, "k_fill_int", k_fill_int, grid1(N), dim3(256), ctx->ix_len, N, 0))
+               ? rc
+               : launch(ctx, "k_loop_len_ch", k_loop_len_ch, grid1(N), dim3(256), (const int*)ctx->ix_fwd,
+                        (const int*)ctx->ix_rev, (const int*)ctx->ix_lab[0], (const int*)ctx->ix_d[0],
+                        (const int*)ctx->ix_chain, N, ctx->ix_len);
   return launch(ctx, "k_loop_len", k_loop_len, grid1(N), dim3(256), (const int*)ctx->ix_rev,
                 (const int*)ctx->ix_lab[0], (const int*)ctx->ix_d[0], N, ctx->ix_len);
+}
+
+// The reference's loop-region eliminations (tangle.c:947-1015) walk closed
+// loops only (a chain's -1 link would be followed).
+int ix_check_regions(ov_ctx* ctx, const ov_topology* tp) {
+  if (ctx->ix_has_pins &&
+      (tp->eliminate_origin_loops || tp->eliminate_zaxis_loops || tp->eliminate_outer_loops))
+    return fail(ctx, OV_E_UNSUPPORTED, "loop eliminations near the origin / the z axis walk closed "
+                "loops only (the reference follows a chain's missing link)");
+  return OV_OK;
 }
 
 int ix_eliminate(ov_ctx* ctx, const ov_topology* tp) {
   const int N = ctx->ix_N;
   if (N == 0) return OV_OK;
   int rc;
+  if ((rc = ix_check_regions(ctx, tp))) return rc;
   if ((rc = ix_label_loops(ctx))) return rc;
   const int mask = (tp->eliminate_origin_loops ? 1 : 0) | (tp->eliminate_zaxis_loops ? 2 : 0) |
                    (tp->eliminate_outer_loops ? 4 : 0);
@@ -1713,19 +1777,30 @@ int ix_to_stream(ov_ctx* ctx) {
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, ctx->ix_len, ctx->ix_off, N, ctx->stream));
   if ((rc = cub_temp(ctx, sb))) return rc;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->t_cub, sb, ctx->ix_len, ctx->ix_off, N, ctx->stream));
-  int last[2] = {0, 0};
+  int last[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(&last[0], ctx->ix_off + (N - 1), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipMemcpyAsync(&last[1], ctx->ix_len + (N - 1), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (ctx->ix_has_pins)
+    HIPCHK(hipMemcpyAsync(&last[2], ctx->ix_state + 4, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   const int n = last[0] + last[1];
   ctx->n = n;
-  ctx->n_seg = n;   // closed loops only
+  ctx->n_seg = n - last[2];   // a chain's last point starts no segment
+  ctx->has_pins = ctx->ix_has_pins;
   if ((rc = ensure_capacity(ctx, std::max(n, 1)))) return rc;
   ctx->cap = ctx->buf_cap;
+  if (ctx->has_pins && ctx->pin_cap < ctx->cap) {
+    if ((rc = dalloc(ctx, (void**)&ctx->pin, sizeof(int) * ctx->cap))) return rc;
+    ctx->pin_cap = ctx->cap;
+  }
   if ((rc = launch(ctx, "k_stream_pos", k_stream_pos, grid1(N), dim3(256), (const int*)ctx->ix_lab[0],
                    (const int*)ctx->ix_d[0], (const int*)ctx->ix_off, N, ctx->ix_pos, ctx->ix_act)))
     return rc;
   if (n == 0) return OV_OK;
+  if (ctx->has_pins &&
+      (rc = launch(ctx, "k_gather_pin", k_gather_pin, grid1(n), dim3(256), (const int*)ctx->ix_pin,
+                   (const int*)ctx->ix_fwd, (const int*)ctx->ix_act, n, ctx->pin)))
+    return rc;
   return launch(ctx, "k_gather_stream", k_gather_stream, grid1(n), dim3(256), (const double*)ctx->ix_X,
                 (long)ctx->ix_alloc, (const int*)ctx->ix_fwd, (const int*)ctx->ix_rev,
                 (const int*)ctx->ix_act, (const int*)ctx->ix_pos, n, ctx->cap, ctx->X0, ctx->fwd,
@@ -2625,7 +2700,7 @@ int ov_gpu_run(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps
   if (flags & OV_USE_CALLER_K1) return fail(ctx, OV_E_INVALID, "ov_gpu_run evaluates k1 itself");
   HIPCHK(hipSetDevice(ctx->device));
   int rc = ix_prepare(ctx, t, p);
-  if (rc) return rc;
+  if (rc || (rc = ix_check_regions(ctx, tp))) return rc;
   if (ctx->ix_N == 0 || nsteps == 0) return OV_OK;
   if (!t->vels || !t->vs || !t->tangents || !t->normals || !t->dxi)
     return fail(ctx, OV_E_INVALID, "null tangle arrays");
@@ -2648,10 +2723,7 @@ int ov_gpu_run(ov_ctx* ctx, ov_tangle_state* t, double t0, double dt, int nsteps
     if ((rc = rk4_device(ctx, time, dt, p->rk_variant, false))) return rc;
     if ((rc = ix_scatter3(ctx, ctx->K[0], ctx->ix_vels)) || (rc = ix_scatter3(ctx, ctx->vs1, ctx->ix_vs)))
       return rc;
-    if ((rc = launch(ctx, "k_geometry", k_geometry, grid1(ctx->n), dim3(256), (const double*)ctx->X0,
-                     (const int*)ctx->fwd, (const int*)ctx->rev, ctx->n, ctx->cap, ctx->box, ctx->ph, 1,
-                     ctx->T, ctx->Nn, ctx->dxi, ctx->lia)))
-      return rc;
+    if ((rc = geometry(ctx, ctx->X0, true, ctx->T, ctx->Nn, ctx->dxi))) return rc;
     if ((rc = ix_scatter3(ctx, ctx->X0, ctx->ix_X)) || (rc = ix_scatter3(ctx, ctx->T, ctx->ix_T)) ||
         (rc = ix_scatter3(ctx, ctx->Nn, ctx->ix_N3)) ||
         (rc = launch(ctx, "k_scatter1", k_scatter1, grid1(ctx->n), dim3(256), (const double*)ctx->dxi,

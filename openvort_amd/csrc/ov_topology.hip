@@ -9,9 +9,15 @@
 // stride `cap`), status[k] < 0 for EMPTY points, links fwd / rev as caller
 // indices.  New points take the slot the reference's get_tangle_next_free
 // would give them, so indices, links and positions follow the reference.
+// Tangles with points pinned on walls carry open chains (a pinned point
+// may lack one link) and a pin code per index (ov_walls.hip's codes, with
+// the forward-index bit kept up to date as links change); their geometry
+// is the walls' (step_node with mirrors, one-sided and neighbour-tangent
+// stencils).
 #include <hip/hip_runtime.h>
 
 #include "ov_device.h"
+#include "ov_walls.hip"
 
 namespace ov {
 
@@ -22,7 +28,8 @@ struct TopoArrays {
   double* dxi;
   int* fwd;
   int* rev;
-  int* status;    // OV_EMPTY / OV_FREE
+  int* status;    // OV_EMPTY / OV_FREE / OV_PINNED / OV_PINNED_SLIP
+  int* pin;       // pin codes (walls), nullptr when no point is pinned
   long cap;
   int N;          // the tangle's capacity (indices 0..N-1)
 };
@@ -98,6 +105,54 @@ __device__ __forceinline__ void refresh_geometry(const TopoArrays& A, const BoxD
   if (lane < naff) topo_tn(A, box, aff[lane]);
   __threadfence_block();
   __syncthreads();
+}
+
+// The same refresh with walls (the geometry of update_tangents_normals
+// with pinned points: dxi, then every point but those next to a PINNED one,
+// then those from their neighbours' s').  A point's s', s'' depend on the
+// points and dxi up to four links away (the one-sided stencil of a PINNED
+// point), and those next to a PINNED point on its s': the caller lists
+// every point within six links of a change.  The forward-index bit of a
+// pinned point's code follows its current forward link.  Returns false
+// (on every lane) when a walk does not end.
+__device__ __forceinline__ bool refresh_geometry_walls(const TopoArrays& A, const BoxDev& box,
+                                                       const int* aff, int naff) {
+  const int lane = threadIdx.x;
+  const int s = lane < naff ? aff[lane] : -1;
+  if (s >= 0 && A.pin[s] >= 0) A.pin[s] = (A.pin[s] & 15) | (A.fwd[s] > 0 ? 16 : 0);
+  __threadfence_block();
+  __syncthreads();
+  bool ok = true;
+  if (s >= 0) ok = wall_dxi_point(s, A.X, A.cap, A.fwd, A.rev, A.pin, box, A.dxi);
+  __threadfence_block();
+  __syncthreads();
+  const bool herm = s >= 0 && wall_hermite(s, A.fwd, A.rev, A.pin);
+  for (int phase = 1; phase <= 2; ++phase) {
+    if (s >= 0 && herm == (phase == 2)) {
+      double p0[3], pn[3], pp[3], t[3], nm[3];
+      load3(A.X, A.cap, s, p0);
+      bool w = wall_walk(s, 1, A.X, A.cap, A.fwd, A.rev, A.pin, box, pn);
+      w = wall_walk(s, -1, A.X, A.cap, A.fwd, A.rev, A.pin, box, pp) && w;
+      if (w && wall_tn_point(s, herm, p0, pn, pp, A.X, A.cap, A.fwd, A.rev, A.pin, box, A.T, A.dxi,
+                             t, nm)) {
+        store3(A.T, A.cap, s, t);
+        store3(A.Nn, A.cap, s, nm);
+      } else {
+        ok = false;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+  return __builtin_amdgcn_ballot_w64(!ok) == 0;
+}
+
+// Whether point s takes the neighbour-tangent stencil (not PINNED, a linked
+// neighbour PINNED), on the index-space arrays.
+__device__ __forceinline__ bool topo_hermite(const TopoArrays& A, int s) {
+  if (A.status[s] == OV_PINNED) return false;
+  const int f = A.fwd[s], r = A.rev[s];
+  return (f >= 0 && A.status[f] == OV_PINNED) || (r >= 0 && A.status[r] == OV_PINNED);
 }
 
 // remove_point (tangle.c:1017-1066) with merge direction dir (+1: merge into
@@ -220,14 +275,18 @@ __device__ __forceinline__ int first_empty(const TopoArrays& A, int from) {
 // from the current state, the first one that changes anything is applied,
 // and the scan resumes after it.
 // state: [0] next_free (in/out), [1] points added, [2] 0 ok / 1 no EMPTY slot
-// left (the reference would expand_tangle; nothing past that point is done),
-// [3] changes made.
+// left (the reference would expand_tangle; nothing past that point is done)
+// / 2 a removal left two neighbouring points between PINNED points, or a
+// walk that does not end (the reference's update_tangent_normal would
+// recurse without end; nothing past that point is done), [3] changes made.
 __global__ void __launch_bounds__(64) k_remesh(TopoArrays A, BoxDev box, double min_dist,
                                                double max_dist, double max_curv, double dl_max,
                                                int* __restrict__ state) {
   const int lane = threadIdx.x;
-  __shared__ int aff[8];
+  __shared__ int aff[16];
   __shared__ int n_aff;
+  __shared__ int bad;
+  if (lane == 0) bad = 0;
   int next_free = state[0];
   int low = 0;          // no EMPTY index below it
   int added = 0, changes = 0;
@@ -258,16 +317,32 @@ __global__ void __launch_bounds__(64) k_remesh(TopoArrays A, BoxDev box, double 
       if (lane == 0) {
         const int prev = A.rev[c], next = A.fwd[c];
         remove_point_ref(A, c, dec == 1 ? 1 : -1);
-        // windows containing prev or next: two links either side
-        const int r1 = A.rev[prev], r2 = A.rev[r1], f1 = A.fwd[next], f2 = A.fwd[f1];
-        aff[0] = r2;
-        aff[1] = r1;
-        aff[2] = prev;
-        aff[3] = next;
-        aff[4] = f1;
-        aff[5] = f2;
+        if (!A.pin) {
+          // windows containing prev or next: two links either side
+          const int r1 = A.rev[prev], r2 = A.rev[r1], f1 = A.fwd[next], f2 = A.fwd[f1];
+          aff[0] = r2;
+          aff[1] = r1;
+          aff[2] = prev;
+          aff[3] = next;
+          aff[4] = f1;
+          aff[5] = f2;
+          naff = 6;
+        } else {
+          // six links either side, stopping at chain ends
+          naff = 0;
+          for (int x = prev, q = 0; q < 7 && x >= 0; ++q, x = A.rev[x]) aff[naff++] = x;
+          for (int x = next, q = 0; q < 7 && x >= 0; ++q, x = A.fwd[x]) aff[naff++] = x;
+          // two neighbouring points next to PINNED ones (or a chain end there)
+          int b = 0;
+          for (int q = 0; q < naff; ++q) {
+            const int x = aff[q];
+            if (!topo_hermite(A, x)) continue;
+            const int f = A.fwd[x], r = A.rev[x];
+            if (f < 0 || r < 0 || topo_hermite(A, f) || topo_hermite(A, r)) b = 1;
+          }
+          bad = b;
+        }
       }
-      naff = 6;
       low = min(low, c);
     } else {
       // get_tangle_next_free (tangle.c:729-757) without expansion
@@ -300,16 +375,24 @@ __global__ void __launch_bounds__(64) k_remesh(TopoArrays A, BoxDev box, double 
         A.fwd[slot] = next;
         A.fwd[c] = slot;
         A.rev[next] = slot;
-        const int r1 = A.rev[c], r2 = A.rev[r1], f1 = A.fwd[next], f2 = A.fwd[f1];
-        aff[0] = r2;
-        aff[1] = r1;
-        aff[2] = c;
-        aff[3] = slot;
-        aff[4] = next;
-        aff[5] = f1;
-        aff[6] = f2;
+        if (A.pin) A.pin[slot] = -1;
+        if (!A.pin) {
+          const int r1 = A.rev[c], r2 = A.rev[r1], f1 = A.fwd[next], f2 = A.fwd[f1];
+          aff[0] = r2;
+          aff[1] = r1;
+          aff[2] = c;
+          aff[3] = slot;
+          aff[4] = next;
+          aff[5] = f1;
+          aff[6] = f2;
+          naff = 7;
+        } else {
+          naff = 0;
+          for (int x = c, q = 0; q < 7 && x >= 0; ++q, x = A.rev[x]) aff[naff++] = x;
+          aff[naff++] = slot;
+          for (int x = next, q = 0; q < 7 && x >= 0; ++q, x = A.fwd[x]) aff[naff++] = x;
+        }
       }
-      naff = 7;
       ++added;
     }
     ++changes;
@@ -325,7 +408,17 @@ __global__ void __launch_bounds__(64) k_remesh(TopoArrays A, BoxDev box, double 
     }
     __threadfence_block();
     __syncthreads();
-    refresh_geometry(A, box, aff, n_aff);
+    if (!A.pin) {
+      refresh_geometry(A, box, aff, n_aff);
+    } else if (bad || !refresh_geometry_walls(A, box, aff, n_aff)) {
+      if (lane == 0) {
+        state[0] = next_free;
+        state[1] = added;
+        state[2] = 2;
+        state[3] = changes;
+      }
+      return;
+    }
     k = c + 1;
   }
   if (lane == 0) {
@@ -414,6 +507,73 @@ __global__ void __launch_bounds__(256) k_loop_len(const int* __restrict__ rev,
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= N) return;
   len[k] = (lab[k] == k) ? d[rev[k]] + 1 : 0;
+}
+
+// With open chains (points pinned on walls) the forward pass gives each
+// point the lowest index of its chain's remainder; a second pass on the
+// reverse links completes the chain's lowest index (for a loop it changes
+// nothing).
+__global__ void __launch_bounds__(256) k_jmp_rev(const int* __restrict__ rev,
+                                                 const int* __restrict__ lab, int N,
+                                                 int* __restrict__ jmp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) jmp[k] = lab[k] >= 0 ? rev[k] : -1;
+}
+// chain[lab] = 1 for a component with an end (a point without a reverse
+// link, its head), counted in *n_chains.
+__global__ void __launch_bounds__(256) k_chain_flag(const int* __restrict__ rev,
+                                                    const int* __restrict__ lab, int N,
+                                                    int* __restrict__ chain, int* __restrict__ n_chains) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N || lab[k] < 0 || rev[k] >= 0) return;
+  chain[lab[k]] = 1;
+  atomicAdd(n_chains, 1);
+}
+// Ranks with chains: a chain is ranked from its head (the stream walks it
+// from there), a loop from its lowest index.
+__global__ void __launch_bounds__(256) k_rank_init_ch(const int* __restrict__ rev,
+                                                      const int* __restrict__ lab,
+                                                      const int* __restrict__ chain, int N,
+                                                      int* __restrict__ d, int* __restrict__ jmp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  if (lab[k] < 0) {
+    d[k] = 0;
+    jmp[k] = -1;
+    return;
+  }
+  const bool head = chain[lab[k]] ? rev[k] < 0 : lab[k] == k;
+  d[k] = head ? 0 : 1;
+  jmp[k] = head ? -1 : rev[k];
+}
+// Component sizes at the label (len zeroed before): a loop's from its lowest
+// index, a chain's from its last point.  eliminate_small_loops counts a
+// chain of L points as L - 1 steps too (tangle.c:899-922: forward to the
+// wall, then back from the start point to the other end).
+__global__ void __launch_bounds__(256) k_loop_len_ch(const int* __restrict__ fwd,
+                                                     const int* __restrict__ rev,
+                                                     const int* __restrict__ lab,
+                                                     const int* __restrict__ d,
+                                                     const int* __restrict__ chain, int N,
+                                                     int* __restrict__ len) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N || lab[k] < 0) return;
+  if (chain[lab[k]]) {
+    if (fwd[k] < 0) len[lab[k]] = d[k] + 1;
+  } else if (lab[k] == k) {
+    len[k] = d[rev[k]] + 1;
+  }
+}
+// Stream pin codes: the index space's, with the forward-index bit of the
+// caller's current forward link.
+__global__ void __launch_bounds__(256) k_gather_pin(const int* __restrict__ pin,
+                                                    const int* __restrict__ fwd,
+                                                    const int* __restrict__ act, int n,
+                                                    int* __restrict__ pin_s) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int k = act[s];
+  pin_s[s] = pin[k] < 0 ? -1 : ((pin[k] & 15) | (fwd[k] > 0 ? 16 : 0));
 }
 
 // Loops to delete (eliminate_small_loops, tangle.c:878-945 and the
@@ -521,8 +681,8 @@ __global__ void __launch_bounds__(256) k_gather_stream(const double* __restrict_
   double p[3];
   load3(Xi, icap, k, p);
   store3(Xs, cap, s, p);
-  fwd_s[s] = pos[fwd[k]];
-  rev_s[s] = pos[rev[k]];
+  fwd_s[s] = fwd[k] >= 0 ? pos[fwd[k]] : -1;
+  rev_s[s] = rev[k] >= 0 ? pos[rev[k]] : -1;
 }
 // A stream-order plane set scattered to the index space.
 __global__ void __launch_bounds__(256) k_scatter3(const double* __restrict__ Ps, long cap,

@@ -11,6 +11,7 @@
 // off such an end turns back along the chain and mirrors the result in the
 // point's wall (step_node, tangle.c:176-191).
 
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include "ov_device.h"
@@ -126,21 +127,102 @@ __device__ __forceinline__ bool wall_hermite(int s, const int* __restrict__ fwd,
   return (f >= 0 && pin_is_pinned(pin[f])) || (r >= 0 && pin_is_pinned(pin[r]));
 }
 
-// dxi[s] = |s - minimum image of step_node(s, +1)| (initialize_dxi).
+// dxi[s] = |s - minimum image of step_node(s, +1)| (initialize_dxi); false
+// when the walk does not end.
+__device__ inline bool wall_dxi_point(int s, const double* __restrict__ P, long cap,
+                                      const int* __restrict__ fwd, const int* __restrict__ rev,
+                                      const int* __restrict__ pin, const BoxDev& box,
+                                      double* __restrict__ dxi) {
+  double p0[3], p1[3];
+  load3(P, cap, s, p0);
+  if (!wall_walk(s, 1, P, cap, fwd, rev, pin, box, p1)) {
+    dxi[s] = __builtin_nan("");
+    return false;
+  }
+  dxi[s] = chord(p0, p1, box);
+  return true;
+}
+
+// s', s'' of point s (update_tangent_normal with walls): the neighbour-
+// tangent stencil when herm (reading the neighbours' s' from T), the
+// one-sided stencil of a PINNED point, the free stencil through step_node /
+// arc_length otherwise.  p0, pn, pp: s and its walked neighbours (+1, -1).
+// false when a walk does not end.
+__device__ inline bool wall_tn_point(int s, bool herm, const double p0[3], const double pn[3],
+                                     const double pp[3], const double* __restrict__ P, long cap,
+                                     const int* __restrict__ fwd, const int* __restrict__ rev,
+                                     const int* __restrict__ pin, const BoxDev& box,
+                                     const double* __restrict__ T, const double* __restrict__ dxi,
+                                     double t[3], double nm[3]) {
+  bool ok = true;
+  if (herm) {
+    // tangle.c:364-430: Hermite-type stencil from the neighbours' tangents t1,
+    // t-1 and ds = (s0 seen from s_j) - s_j for j = +-1, a = arc(s, 1),
+    // b = -arc(s, -1); the coefficients are the reference's (its geometry at
+    // walls is defined by them)
+    const int f = fwd[s], r = rev[s];
+    double t1[3], tm1[3], w1[3], wm1[3], ds1[3], dsm1[3];
+    load3(T, cap, f, t1);
+    load3(T, cap, r, tm1);
+    pwrap(pn, p0, box, w1);
+    pwrap(pp, p0, box, wm1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ds1[i] = w1[i] - pn[i];
+      dsm1[i] = wm1[i] - pp[i];
+    }
+    const double a = wall_arc(s, 1, dxi, fwd, rev);
+    const double b = -wall_arc(s, -1, dxi, fwd, rev);
+    const double D = a * a - 2 * a * b + b * b;
+    const double D3 = a * a * a - 3 * a * a * b + 3 * a * b * b - b * b * b;
+    const double ct1 = -b * b / D, ctm1 = -a * a / D;
+    const double ct_ds1 = 2 * b * b * (-2 * a + b) / (a * D3);
+    const double ct_dsm1 = 2 * a * a * (-a + 2 * b) / (b * D3);
+    const double cn_ds1 = 2 * b * (8 * a * a - a * b - b * b) / (a * a * D3);
+    const double cn_dsm1 = 2 * a * (a * a + a * b - 8 * b * b) / (b * b * D3);
+    const double cn_t1 = 2 * b * (2 * a + b) / (a * D);
+    const double cn_tm1 = 2 * a * (a + 2 * b) / (b * D);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      t[i] = ct1 * t1[i] + ctm1 * tm1[i] + ct_ds1 * ds1[i] + ct_dsm1 * dsm1[i];
+      nm[i] = cn_t1 * t1[i] + cn_tm1 * tm1[i] + cn_ds1 * ds1[i] + cn_dsm1 * dsm1[i];
+    }
+    return true;
+  }
+  const int pc = pin[s];
+  double x[4], pj[4][3];
+  if (pin_is_pinned(pc)) {
+    // one-sided stencil on the nodes 1..4 links away (tangle.c:275-362),
+    // forward when the caller's forward index is > 0
+    const int dir = ((pc >> 4) & 1) ? 1 : -1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ok = wall_walk(s, dir * (j + 1), P, cap, fwd, rev, pin, box, pj[j]) && ok;
+      x[j] = dir * wall_arc(s, dir * (j + 1), dxi, fwd, rev);
+    }
+  } else {
+    // the free stencil (tangle.c:432-479) through step_node / arc_length
+    ok = wall_walk(s, 2, P, cap, fwd, rev, pin, box, pj[0]) && ok;
+    pj[1][0] = pn[0], pj[1][1] = pn[1], pj[1][2] = pn[2];
+    pj[2][0] = pp[0], pj[2][1] = pp[1], pj[2][2] = pp[2];
+    ok = wall_walk(s, -2, P, cap, fwd, rev, pin, box, pj[3]) && ok;
+    x[0] = wall_arc(s, 2, dxi, fwd, rev);
+    x[1] = wall_arc(s, 1, dxi, fwd, rev);
+    x[2] = -wall_arc(s, -1, dxi, fwd, rev);
+    x[3] = -wall_arc(s, -2, dxi, fwd, rev);
+  }
+  if (!ok) return false;
+  stencil5(x, pj, p0, box, t, nm);
+  return true;
+}
+
 __global__ void __launch_bounds__(256) k_wall_dxi(const double* __restrict__ P, const int* __restrict__ fwd,
                                                   const int* __restrict__ rev, const int* __restrict__ pin,
                                                   int n, long cap, BoxDev box, double* __restrict__ dxi,
                                                   int* __restrict__ flag) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
-  double p0[3], p1[3];
-  load3(P, cap, s, p0);
-  if (!wall_walk(s, 1, P, cap, fwd, rev, pin, box, p1)) {
-    atomicOr(flag, 1);
-    dxi[s] = __builtin_nan("");
-    return;
-  }
-  dxi[s] = chord(p0, p1, box);
+  if (!wall_dxi_point(s, P, cap, fwd, rev, pin, box, dxi)) atomicOr(flag, 1);
 }
 
 // s', s'' (when compute_geom) and the local induction of point s.  Phase 1:
@@ -173,66 +255,9 @@ __global__ void __launch_bounds__(256) k_wall_geometry(const double* __restrict_
   if (!compute_geom) {
     load3(T, cap, s, t);
     load3(Nn, cap, s, nm);
-  } else if (herm) {
-    // tangle.c:364-430: Hermite-type stencil from the neighbours' tangents t1,
-    // t-1 and ds = (s0 seen from s_j) - s_j for j = +-1, a = arc(s, 1),
-    // b = -arc(s, -1); the coefficients are the reference's (its geometry at
-    // walls is defined by them)
-    const int f = fwd[s], r = rev[s];
-    double t1[3], tm1[3], w1[3], wm1[3], ds1[3], dsm1[3];
-    load3(T, cap, f, t1);
-    load3(T, cap, r, tm1);
-    pwrap(pn, p0, box, w1);
-    pwrap(pp, p0, box, wm1);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      ds1[i] = w1[i] - pn[i];
-      dsm1[i] = wm1[i] - pp[i];
-    }
-    const double a = wall_arc(s, 1, dxi, fwd, rev);
-    const double b = -wall_arc(s, -1, dxi, fwd, rev);
-    const double D = a * a - 2 * a * b + b * b;
-    const double D3 = a * a * a - 3 * a * a * b + 3 * a * b * b - b * b * b;
-    const double ct1 = -b * b / D, ctm1 = -a * a / D;
-    const double ct_ds1 = 2 * b * b * (-2 * a + b) / (a * D3);
-    const double ct_dsm1 = 2 * a * a * (-a + 2 * b) / (b * D3);
-    const double cn_ds1 = 2 * b * (8 * a * a - a * b - b * b) / (a * a * D3);
-    const double cn_dsm1 = 2 * a * (a * a + a * b - 8 * b * b) / (b * b * D3);
-    const double cn_t1 = 2 * b * (2 * a + b) / (a * D);
-    const double cn_tm1 = 2 * a * (a + 2 * b) / (b * D);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      t[i] = ct1 * t1[i] + ctm1 * tm1[i] + ct_ds1 * ds1[i] + ct_dsm1 * dsm1[i];
-      nm[i] = cn_t1 * t1[i] + cn_tm1 * tm1[i] + cn_ds1 * ds1[i] + cn_dsm1 * dsm1[i];
-    }
-  } else {
-    const int pc = pin[s];
-    double x[4], pj[4][3];
-    if (pin_is_pinned(pc)) {
-      // one-sided stencil on the nodes 1..4 links away (tangle.c:275-362),
-      // forward when the caller's forward index is > 0
-      const int dir = ((pc >> 4) & 1) ? 1 : -1;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        ok = wall_walk(s, dir * (j + 1), P, cap, fwd, rev, pin, box, pj[j]) && ok;
-        x[j] = dir * wall_arc(s, dir * (j + 1), dxi, fwd, rev);
-      }
-    } else {
-      // the free stencil (tangle.c:432-479) through step_node / arc_length
-      ok = wall_walk(s, 2, P, cap, fwd, rev, pin, box, pj[0]) && ok;
-      pj[1][0] = pn[0], pj[1][1] = pn[1], pj[1][2] = pn[2];
-      pj[2][0] = pp[0], pj[2][1] = pp[1], pj[2][2] = pp[2];
-      ok = wall_walk(s, -2, P, cap, fwd, rev, pin, box, pj[3]) && ok;
-      x[0] = wall_arc(s, 2, dxi, fwd, rev);
-      x[1] = wall_arc(s, 1, dxi, fwd, rev);
-      x[2] = -wall_arc(s, -1, dxi, fwd, rev);
-      x[3] = -wall_arc(s, -2, dxi, fwd, rev);
-    }
-    if (!ok) {
-      atomicOr(flag, 1);
-      return;
-    }
-    stencil5(x, pj, p0, box, t, nm);
+  } else if (!wall_tn_point(s, herm, p0, pn, pp, P, cap, fwd, rev, pin, box, T, dxi, t, nm)) {
+    atomicOr(flag, 1);
+    return;
   }
   if (compute_geom) {
     store3(T, cap, s, t);

@@ -1,0 +1,204 @@
+"""Topology edits of tangles with vortex lines pinned on walls against the
+reference's own functions (src/tangle.c compiled into oracle/_ref): remesh
+next to chain ends (tangle.c:824-873, 1017-1160, each change followed by
+update_tangents_normals with the walls' stencils), eliminate_small_loops on
+open chains (tangle.c:878-945: a chain of L points counts L - 1 steps, as a
+loop does) and the main loop body (main.c:112-117) on the device.
+
+Box 0.1 cm, periodic in x and y, mirror walls at z = 0 and z = 0.1.  Links,
+status and next_free must be identical; positions within 1e-15 cm (the
+points remesh moves or inserts come from s' and s'', which the device's
+stencils give to ~1e-12 relative against the reference's); dxi 1e-13
+relative.  Against the same steps driven from the host through the
+single-call entry points the resident loop is bit-identical.
+"""
+import numpy as np
+import pytest
+
+from openvort_amd import tangle as T
+from openvort_amd.gpu import OV_E_UNSUPPORTED, OVError
+from oracle.oracle import Reference
+from tests.test_gpu_walls import IMAGES, L, WALLS, copy, padded, params, wall_line
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not Reference.available(), reason="reference library not built")]
+
+POS_TOL = 1e-15
+DL_MIN, DL_MAX = 1.7e-3, 2.6e-3
+
+
+def _line_z(rng, n_mid, short_ends):
+    """z of a wall-to-wall line: short segments next to both walls (short_ends)
+    or a long first and last segment, jittered uniform spacing between."""
+    if short_ends:
+        head, tail = [0.0, 0.8e-3, 1.6e-3], [L - 1.6e-3, L - 0.8e-3, L]
+    else:
+        head, tail = [0.0], [L]
+    a, b = (head[-1] + 2.2e-3, tail[0] - 2.2e-3) if short_ends else (3.0e-3, L - 3.0e-3)
+    mid = np.linspace(a, b, n_mid)
+    mid[1:-1] += rng.uniform(-0.25, 0.25, n_mid - 2) * (b - a) / (n_mid - 1)
+    return np.concatenate([head, mid, tail])
+
+
+def _line(rng, z, x0, y0, amp=1.5e-3):
+    x = np.full(len(z), x0)
+    y = np.full(len(z), y0)
+    for m in (1, 2):
+        a, b = rng.uniform(-amp, amp, 2) / m
+        x += a * np.sin(np.pi * m * z / L)
+        y += b * np.sin(np.pi * m * z / L)
+    return np.stack([x, y, z], 1)
+
+
+def remesh_wall_tangle(seed=5, status=T.PINNED, small=(), holes=200):
+    """Two lines with short segments at the walls, one with long segments
+    there, a ring whose spacing remesh thins, and optional small loops and
+    chains (small: (kind, npts) with kind 'ring', 'chain' (wall to wall) or
+    'arc' (a half circle standing on the bottom wall, 2e-3 cm segments))."""
+    rng = np.random.default_rng(seed)
+    b = T.TangleBuilder()
+    for q, short in enumerate((True, False, True)):
+        z = _line_z(rng, 40, short)
+        b.add_chain(_line(rng, z, *rng.uniform(0.02, 0.08, 2)), (status, T.Z_L), (status, T.Z_H))
+    b.add_ring((0.05, 0.05, 0.05), (0.3, 0.2, 0.93), 0.012, 60)
+    for kind, npts in small:
+        if kind == "ring":
+            b.add_ring(tuple(rng.uniform(0.02, 0.08, 3)), (0.0, 0.0, 1.0), 1.5e-3, npts)
+        elif kind == "arc":
+            th = np.pi * np.arange(npts) / (npts - 1)
+            r = 1e-3 / np.sin(np.pi / (2 * (npts - 1)))
+            x0, y0 = rng.uniform(0.02, 0.08, 2)
+            pts = np.stack([x0 + r * np.cos(th), np.full(npts, y0), r * np.sin(th)], 1)
+            pts[[0, -1], 2] = 0.0
+            b.add_chain(pts, (status, T.Z_L), (status, T.Z_L))
+        else:
+            b.add_chain(wall_line(rng, npts, *rng.uniform(0.02, 0.08, 2)), (status, T.Z_L),
+                        (status, T.Z_H))
+    t = b.build((0, 0, 0), (L, L, L), WALLS, IMAGES, holes=holes, seed=seed)
+    t.next_free = 0
+    return padded(t)
+
+
+def _same_topology(a, b):
+    np.testing.assert_array_equal(b.status[:, 0], a.status[:, 0])
+    np.testing.assert_array_equal(b.connections, a.connections)
+    assert a.next_free == b.next_free
+    act = a.active
+    return np.abs(a.vnodes[act] - b.vnodes[act]).max()
+
+
+@pytest.mark.parametrize("status", [T.PINNED, T.PINNED_SLIP])
+def test_wall_remesh_matches_reference(ctx, status):
+    p = params(dl_max=DL_MAX)
+    R = Reference()
+    R.set_params(p)
+    a = remesh_wall_tangle(status=status)
+    R.update_tangents_normals(a)
+    b = copy(a)
+    n0 = len(a.active)
+    R.remesh(a, DL_MIN, DL_MAX, need_free=n0)
+    ctx.remesh(b, p, DL_MIN, DL_MAX)
+    d = _same_topology(a, b)
+    act = a.active
+    print(f"status {status}: remesh {n0} -> {len(act)} points, position diff {d:.3g}", flush=True)
+    assert len(act) != n0
+    assert d < POS_TOL
+    np.testing.assert_allclose(b.dxi[act], a.dxi[act], rtol=1e-13, atol=0)
+    # the chain ends are still pinned and still end their chains
+    ends = act[a.status[act, 0] != T.FREE]
+    assert len(ends) == 6
+    assert np.all((b.connections[ends, 0] < 0) | (b.connections[ends, 1] < 0))
+
+
+def test_wall_remesh_changes_next_to_the_walls(ctx):
+    """The removals and insertions include points linked to a pinned end:
+    the first free point of a line with short end segments goes, a long
+    segment from a pinned head gets a new point."""
+    p = params(dl_max=DL_MAX)
+    R = Reference()
+    R.set_params(p)
+    t = remesh_wall_tangle()
+    R.update_tangents_normals(t)
+    b = copy(t)
+    ctx.remesh(b, p, DL_MIN, DL_MAX)
+    heads = np.nonzero((t.status[:, 0] == T.PINNED) & (t.connections[:, 1] == -1))[0]
+    changed = [h for h in heads if b.connections[h, 0] != t.connections[h, 0]]
+    removed = [h for h in heads if b.status[t.connections[h, 0], 0] == T.EMPTY]
+    print("heads", len(heads), "with a new neighbour", len(changed), "first point removed", len(removed),
+          flush=True)
+    assert len(removed) >= 1 and len(changed) > len(removed)
+
+
+def test_wall_eliminate_small_loops_matches_reference(ctx):
+    """Chains of 3, 5 and 7 points and loops of 4 to 6 points next to the
+    long lines: components of L points with L - 1 < 5 go."""
+    t = remesh_wall_tangle(small=[("chain", 3), ("chain", 5), ("chain", 7), ("ring", 4), ("ring", 5),
+                                  ("ring", 6)])
+    tp = T.default_topology()
+    a, b = copy(t), copy(t)
+    Reference().eliminate_small_loops(a, tp.small_loop_cutoff)
+    ctx.eliminate_small_loops(b, tp)
+    np.testing.assert_array_equal(b.status[:, 0], a.status[:, 0])
+    np.testing.assert_array_equal(b.connections, a.connections)
+    removed = len(t.active) - len(a.active)
+    print("removed", removed, "points", flush=True)
+    assert removed == 3 + 5 + 4 + 5
+
+
+def test_wall_region_eliminations_unsupported(ctx):
+    t = remesh_wall_tangle()
+    tp = T.default_topology(eliminate_origin_loops=1, eliminate_loops_origin_cutoff=0.01)
+    with pytest.raises(OVError) as e:
+        ctx.eliminate_small_loops(copy(t), tp)
+    assert e.value.code == OV_E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("status,prec", [(T.PINNED, T.PREC_FP64), (T.PINNED_SLIP, T.PREC_FP64),
+                                         (T.PINNED, T.PREC_MIXED)])
+def test_wall_run_matches_host_sequence(ctx, status, prec):
+    """ov_gpu_run over 4 steps equals the same steps driven from the host
+    with the single-call entry points bit for bit: the device rebuilds the
+    stream (chains from their heads) and the pin codes as prepare() does."""
+    p = params(dl_max=DL_MAX, precision=prec, near_radius=0.01)
+    tp = T.default_topology(dl_min=DL_MIN, dl_max=DL_MAX)
+    t = remesh_wall_tangle(seed=21, status=status, small=[("ring", 4), ("arc", 5)])
+    a, b = copy(t), copy(t)
+    dt = 1e-5
+    for k in range(4):
+        ctx.rk4_step(a, k * dt, dt, p)
+        ctx.update_tangents_normals(a)
+        ctx.remesh(a, p, tp.dl_min, tp.dl_max)
+        ctx.eliminate_small_loops(a, tp)
+        ctx.enforce_boundaries(a)
+    ctx.run(b, 0.0, dt, 4, p, tp)
+    d = _same_topology(a, b)
+    act = a.active
+    print(f"run vs host sequence: {len(t.active)} -> {len(act)} points, position diff {d}", flush=True)
+    assert d == 0
+    np.testing.assert_array_equal(b.dxi[act], a.dxi[act])
+    np.testing.assert_array_equal(b.tangents[act], a.tangents[act])
+
+
+def test_wall_run_matches_reference_main_loop(ctx):
+    """Three steps of the reference's loop body with its own functions on the
+    CPU (update_tangle, rk4_step, update_tangents_normals, remesh,
+    eliminate_small_loops, enforce_boundaries) against ov_gpu_run in FP64:
+    identical links and status, positions within 1e-14 cm."""
+    R = Reference()
+    p = params(dl_max=DL_MAX)
+    tp = T.default_topology(dl_min=DL_MIN, dl_max=DL_MAX)
+    R.set_params(p)
+    t = remesh_wall_tangle(seed=34, small=[("ring", 5), ("arc", 5)])
+    a, b = copy(t), copy(t)
+    dt = 1e-5
+    n0 = len(t.active)
+    for k in range(3):
+        R.step(a, k * dt, dt, p)
+        R.update_tangents_normals(a)
+        R.remesh(a, tp.dl_min, tp.dl_max, need_free=60)
+        R.eliminate_small_loops(a, tp.small_loop_cutoff)
+        R.enforce_boundaries(a)
+    ctx.run(b, 0.0, dt, 3, p, tp)
+    d = _same_topology(a, b)
+    print(f"run vs reference loop: {n0} -> {len(a.active)} points, position diff {d}", flush=True)
+    assert d < 1e-14
