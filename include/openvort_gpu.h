@@ -182,7 +182,8 @@ void *ov_ctx_stream(ov_ctx *ctx);
 /* update_tangents_normals (tangle.h:120): writes dxi, tangents, normals.
  *
  * Walls (this call, ov_gpu_velocities and ov_gpu_rk4_step / ov_gpu_rk4_run,
- * direct sum or octree, the topology edits and ov_gpu_run): PINNED and PINNED_SLIP points with pin_wall 0..5
+ * direct sum or octree, the ensembles, the topology edits and ov_gpu_run):
+ * PINNED and PINNED_SLIP points with pin_wall 0..5
  * may end open chains (forward or reverse -1).  Neighbours past a chain end
  * are the mirror images of the chain in that point's wall (step_node,
  * tangle.c:165-198); PINNED points take the one-sided stencil (tangle.c:
@@ -219,8 +220,10 @@ int ov_gpu_rk4_run(ov_ctx *ctx, ov_tangle_state *tangle, double t0, double dt,
 
 /* Ensembles: nsteps RK4 steps (update_tangle + rk4_step each, as
  * ov_gpu_rk4_run) of ntangles independent tangles, params[k] for tangle k
- * (each with its own box, images, flows and friction; all with one precision
- * and RK variant; direct sum only).  Every velocity evaluation of the whole
+ * (each with its own box, images, flows and friction, and its own walls: a
+ * tangle may hold points pinned on walls as ov_gpu_rk4_run takes them; all
+ * with one precision and RK variant; direct sum only).  Every velocity
+ * evaluation of the whole
  * ensemble is one persistent launch over a queue of work units; one RK4 step
  * of the ensemble is captured once as a HIP graph and replayed; unit size
  * and grid are picked by timing candidates on the first call for a layout.

@@ -7,9 +7,13 @@
 // The direct sum is one persistent launch: waves take work units from a
 // queue; a unit is up to U (source tile, image) items of one target tile of
 // one tangle.  Unit partial sums are reduced per target in unit order.
+// Tangles with points pinned on walls carry open chains (a chain's last
+// point starts no segment) and pin codes (ov_walls.hip) in the batch's pin
+// array.
 #include <hip/hip_runtime.h>
 
 #include "ov_device.h"
+#include "ov_walls.hip"
 
 namespace ov {
 
@@ -23,7 +27,7 @@ struct TangleDev {
   int img_base, n_img;   // images (primary first) in the batch image list
   int lia_only;
   int wall_mask;
-  int pad;
+  int pins;              // 1: points pinned on walls (the walls' geometry)
 };
 
 __global__ void __launch_bounds__(256) kb_geometry(const double* __restrict__ P,
@@ -42,7 +46,69 @@ __global__ void __launch_bounds__(256) kb_geometry(const double* __restrict__ P,
   geometry_point(s, P, fwd, rev, cap, tg[k].box, tg[k].ph, compute_geom, T, Nn, dxi, lia);
 }
 
-// Segment breaks (as k_cont) with the tangle's own box; padding slots have none.
+// Geometry of a batch with points pinned on walls, as geometry() does it for
+// one tangle: phase 0 dxi of the tangles with pins; phase 1 the free
+// tangles' geometry and, in the others, every point but those next to a
+// PINNED point; phase 2 those (compute_geom: s', s'' as well as the local
+// induction; otherwise the caller's s', s'' in T, Nn and phase 1 only).
+__global__ void __launch_bounds__(256) kb_wall_geometry(const double* __restrict__ P,
+                                                        const int* __restrict__ fwd,
+                                                        const int* __restrict__ rev,
+                                                        const int* __restrict__ pin, long cap,
+                                                        const int* __restrict__ tid,
+                                                        const TangleDev* __restrict__ tg,
+                                                        int compute_geom, int phase,
+                                                        double* __restrict__ T,
+                                                        double* __restrict__ Nn,
+                                                        double* __restrict__ dxi,
+                                                        double* __restrict__ lia,
+                                                        int* __restrict__ flag, int ntot) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= ntot) return;
+  const int k = tid[s];
+  if (k < 0) return;
+  const TangleDev& t = tg[k];
+  if (!t.pins) {
+    if (phase == 1) geometry_point(s, P, fwd, rev, cap, t.box, t.ph, compute_geom, T, Nn, dxi, lia);
+    return;
+  }
+  if (phase == 0) {
+    if (!wall_dxi_point(s, P, cap, fwd, rev, pin, t.box, dxi)) atomicOr(flag, 1);
+    return;
+  }
+  const bool herm = compute_geom && wall_hermite(s, fwd, rev, pin);
+  if (herm != (phase == 2)) return;
+  double p0[3], pn[3], pp[3];
+  load3(P, cap, s, p0);
+  bool ok = wall_walk(s, 1, P, cap, fwd, rev, pin, t.box, pn);
+  ok = wall_walk(s, -1, P, cap, fwd, rev, pin, t.box, pp) && ok;
+  double tv[3], nm[3];
+  if (ok && !compute_geom) {
+    load3(T, cap, s, tv);
+    load3(Nn, cap, s, nm);
+  } else if (ok) {
+    ok = wall_tn_point(s, herm, p0, pn, pp, P, cap, fwd, rev, pin, t.box, T, dxi, tv, nm);
+  }
+  if (!ok) {
+    atomicOr(flag, 1);
+    return;
+  }
+  if (compute_geom) {
+    store3(T, cap, s, tv);
+    store3(Nn, cap, s, nm);
+  }
+  // lia_velocity (tangle.c:530-551) with the walked neighbours
+  const double lf = chord(p0, pn, t.box), lr = chord(pp, p0, t.box);
+  const double pi = 3.14159265358979323846;
+  const double beta =
+      t.ph.kappa * log(2 * sqrt(lf * lr) / sqrt(sqrt(2.718281828459045)) / t.ph.a0) / 4 / pi;
+  const double v[3] = {(tv[1] * nm[2] - tv[2] * nm[1]) * beta, (tv[2] * nm[0] - tv[0] * nm[2]) * beta,
+                       (tv[0] * nm[1] - tv[1] * nm[0]) * beta};
+  store3(lia, cap, s, v);
+}
+
+// Segment breaks (as k_cont) with the tangle's own box; padding slots have
+// none; a chain's last point (no forward link) is 2: no segment, no record.
 __global__ void __launch_bounds__(256) kb_cont(const double* __restrict__ P,
                                                const int* __restrict__ fwd, long cap,
                                                const int* __restrict__ tid,
@@ -54,16 +120,10 @@ __global__ void __launch_bounds__(256) kb_cont(const double* __restrict__ P,
   int nc = 0;
   if (s < ntot) {
     const int k = tid[s];
-    if (k >= 0) {
-      const int f = fwd[s];
-      double a[3], pf[3], b[3];
-      load3(P, cap, s, a);
-      load3(P, cap, f, pf);
-      pwrap(a, pf, tg[k].box, b);
-      const bool cont = (f == s + 1) && b[0] == pf[0] && b[1] == pf[1] && b[2] == pf[2];
-      nc = cont ? 0 : 1;
-    }
-    brk[s] = nc;
+    int b = 0;
+    if (k >= 0) b = cont_flag(P, fwd, s, cap, tg[k].box);
+    brk[s] = b;
+    nc = b == 1 ? 1 : 0;
   }
   int tot;
   const int r = block_exscan256(nc, lds, &tot);
@@ -110,7 +170,7 @@ __global__ void __launch_bounds__(256) kb_records(const double* __restrict__ P,
     r0.seg_fwd = -1;
   }
   near[off] = r0;
-  if (brk[s]) {
+  if (brk[s] == 1) {
     const int f = fwd[s];
     double pf[3], b[3];
     load3(P, cap, f, pf);
@@ -124,7 +184,7 @@ __global__ void __launch_bounds__(256) kb_records(const double* __restrict__ P,
     near[off + 1] = r1;
   }
   if (s == pb + t.n - 1) {
-    const int M = t.n + (Es + brk[s] - E0);
+    const int M = t.n + (Es + (brk[s] == 1 ? 1 : 0) - E0);
     tM[k] = M;
     tS[k] = (M + TILE - 1) / TILE;
   }
@@ -330,8 +390,8 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A) {
     L.s = T * TILE + lane;
     L.valid = (L.s - t.pt_base) < t.n;
     load3(A.P, A.cap, L.valid ? L.s : T * TILE, L.r);
-    L.adjA = L.valid ? A.adj[2 * L.s] / TILE : -1;
-    L.adjB = L.valid ? A.adj[2 * L.s + 1] / TILE : -1;
+    L.adjA = L.valid ? adj_tile(A.adj[2 * L.s]) : -1;
+    L.adjB = L.valid ? adj_tile(A.adj[2 * L.s + 1]) : -1;
     const TgtTile tt = A.tgt[T];
     double acc[3] = {0, 0, 0};
     direct_items<MIXED, false>(A.src, A.far, A.near, A.img + t.img_base, t.n_img, t.rec_base / TILE,
@@ -377,6 +437,7 @@ __global__ void __launch_bounds__(256) kb_epilogue(const double* __restrict__ P,
                                                    const double* __restrict__ bs, long cap,
                                                    double time, const int* __restrict__ tid,
                                                    const TangleDev* __restrict__ tg,
+                                                   const int* __restrict__ pin,
                                                    double* __restrict__ vs_out,
                                                    double* __restrict__ vel_out,
                                                    int* __restrict__ flag, int ntot) {
@@ -384,7 +445,8 @@ __global__ void __launch_bounds__(256) kb_epilogue(const double* __restrict__ P,
   if (s >= ntot) return;
   const int k = tid[s];
   if (k < 0) return;
-  epilogue_point(s, P, T, Nn, lia, bs, !tg[k].lia_only, cap, time, tg[k].ph, -1, vs_out, vel_out, flag);
+  epilogue_point(s, P, T, Nn, lia, bs, !tg[k].lia_only, cap, time, tg[k].ph, pin ? pin[s] : -1, vs_out,
+                 vel_out, flag);
 }
 
 __global__ void __launch_bounds__(256) kb_wrap(double* __restrict__ X0, long cap,

@@ -137,7 +137,9 @@ struct ov_ctx {
   TangleDev* b_dtg = nullptr;
   int *b_tid = nullptr, *b_tile_tangle = nullptr, *b_tgt_tangle = nullptr, *b_tM = nullptr,
       *b_tS = nullptr, *b_ucnt = nullptr, *b_urank = nullptr, *b_ubsum = nullptr,
-      *b_ucounts = nullptr, *b_queue = nullptr;
+      *b_ucounts = nullptr, *b_queue = nullptr, *b_pin = nullptr;
+  bool b_pins = false;   // a member has points pinned on walls (pin codes in b_pin)
+  long b_cap_pin = 0;
   ImgDev* b_img = nullptr;
   double* b_upart = nullptr;
   size_t b_upart_bytes = 0;
@@ -437,8 +439,8 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p, bool allo
     if (st == OV_PINNED || st == OV_PINNED_SLIP) {
       if (!allow_pins)
         return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are supported by ov_gpu_velocities, "
-                    "ov_gpu_rk4_step/run, ov_gpu_update_tangents_normals, the topology edits and "
-                    "ov_gpu_run only", k);
+                    "ov_gpu_rk4_step/run, ov_gpu_rk4_run_batch, ov_gpu_update_tangents_normals, the "
+                    "topology edits and ov_gpu_run only", k);
       if (t->status[k].pin_wall < 0 || t->status[k].pin_wall > 5)
         return fail(ctx, OV_E_INVALID, "pinned point %d has pin_wall %d", k, t->status[k].pin_wall);
       pins = true;
@@ -1157,10 +1159,22 @@ int evaluate_batch(ov_ctx* ctx, const double* X, double* Tg, double* Ng, double*
   const int ntot = ctx->b_ntot, rec_tot = ctx->b_rec_tot, n_tgt = ctx->b_n_tgt;
   const int nb = (ntot + 255) / 256, nbu = (n_tgt + 255) / 256;
   int rc;
-  if ((rc = launch(ctx, "kb_geometry", kb_geometry, dim3(nb), dim3(256), X, (const int*)ctx->fwd,
-                   (const int*)ctx->rev, cap, (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, 1, Tg,
-                   Ng, dxi, ctx->lia, ntot)))
-    return rc;
+  if (!ctx->b_pins) {
+    if ((rc = launch(ctx, "kb_geometry", kb_geometry, dim3(nb), dim3(256), X, (const int*)ctx->fwd,
+                     (const int*)ctx->rev, cap, (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, 1, Tg,
+                     Ng, dxi, ctx->lia, ntot)))
+      return rc;
+  } else {
+    for (int phase = 0; phase <= 2; ++phase)
+      if ((rc = launch(ctx, "kb_wall_geometry", kb_wall_geometry, dim3(nb), dim3(256), X,
+                       (const int*)ctx->fwd, (const int*)ctx->rev, (const int*)ctx->b_pin, cap,
+                       (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, 1, phase, Tg, Ng, dxi,
+                       ctx->lia, ctx->flag, ntot)))
+        return rc;
+    // a chain end lacks one adjacent segment: its entry stays -1
+    if ((rc = launch(ctx, "k_fill_int", k_fill_int, grid1(2 * ntot), dim3(256), ctx->adj, 2 * ntot, -1)))
+      return rc;
+  }
   if ((rc = launch(ctx, "kb_cont", kb_cont, dim3(nb), dim3(256), X, (const int*)ctx->fwd, cap,
                    (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, ctx->brk, ctx->rank, ctx->bsum,
                    ntot)))
@@ -1242,7 +1256,8 @@ int evaluate_batch(ov_ctx* ctx, const double* X, double* Tg, double* Ng, double*
     return rc;
   return launch(ctx, "kb_epilogue", kb_epilogue, dim3(nb), dim3(256), X, (const double*)Tg,
                 (const double*)Ng, (const double*)ctx->lia, (const double*)ctx->bs, cap, time,
-                (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg, vs, vel, ctx->flag, ntot);
+                (const int*)ctx->b_tid, (const TangleDev*)ctx->b_dtg,
+                ctx->b_pins ? (const int*)ctx->b_pin : (const int*)nullptr, vs, vel, ctx->flag, ntot);
 }
 
 // update_tangle + rk4_step of every tangle of the batch (X0 device-resident).
@@ -1901,7 +1916,7 @@ void ov_ctx_destroy(ov_ctx* ctx) {
     if (b) hipFree(b);
   void* bb[] = {ctx->b_dtg,  ctx->b_tid,   ctx->b_tile_tangle, ctx->b_tgt_tangle, ctx->b_tM,
                 ctx->b_tS,   ctx->b_ucnt,  ctx->b_urank,       ctx->b_ubsum,      ctx->b_ucounts,
-                ctx->b_queue, ctx->b_img,  ctx->b_upart};
+                ctx->b_queue, ctx->b_img,  ctx->b_upart, ctx->b_pin};
   for (void* b : bb)
     if (b) hipFree(b);
   void* ab[] = {ctx->a_gnb,   ctx->a_gu,    ctx->a_gl,    ctx->a_SB,    ctx->a_W,
@@ -1916,7 +1931,7 @@ void ov_ctx_destroy(ov_ctx* ctx) {
   void* ib[] = {ctx->ix_X, ctx->ix_T, ctx->ix_N3, ctx->ix_dxi, ctx->ix_vels, ctx->ix_vs, ctx->ix_fwd,
                 ctx->ix_rev, ctx->ix_status, ctx->ix_lab[0], ctx->ix_lab[1], ctx->ix_jmp[0],
                 ctx->ix_jmp[1], ctx->ix_d[0], ctx->ix_d[1], ctx->ix_len, ctx->ix_off, ctx->ix_pos,
-                ctx->ix_act, ctx->ix_bits, ctx->ix_all, ctx->ix_state};
+                ctx->ix_act, ctx->ix_bits, ctx->ix_all, ctx->ix_state, ctx->ix_pin, ctx->ix_chain};
   for (void* b : ib)
     if (b) hipFree(b);
   if (ctx->hst) hipHostFree(ctx->hst);
@@ -2087,13 +2102,15 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
   std::vector<std::vector<int>> fwd_l(K), rev_l(K);
   std::vector<ImgDev> imgs;
   std::vector<int> npts(K);
+  std::vector<std::vector<int>> pin_l(K);
   double rnear = 0;
   bool time_dep = false;
+  ctx->b_pins = false;
   for (int k = 0; k < K; ++k) {
     if (params[k].use_BH) return fail(ctx, OV_E_UNSUPPORTED, "batch: octree not supported (tangle %d)", k);
     if (params[k].precision != prec || params[k].rk_variant != rkv)
       return fail(ctx, OV_E_INVALID, "batch: tangles must share precision and RK variant");
-    int rc = prepare(ctx, tangles[k], &params[k]);
+    int rc = prepare(ctx, tangles[k], &params[k], true);
     if (rc) {
       char msg[512];
       snprintf(msg, sizeof msg, "tangle %d: %s", k, ctx->err);
@@ -2108,6 +2125,11 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
     t.n = ctx->n;
     t.lia_only = ctx->lia_only;
     t.wall_mask = ctx->wall_periodic_mask;
+    t.pins = ctx->has_pins ? 1 : 0;
+    if (ctx->has_pins) {
+      pin_l[k] = ctx->pin_s;
+      ctx->b_pins = true;
+    }
     t.img_base = (int)imgs.size();
     t.n_img = (int)ctx->imgs.size();
     imgs.insert(imgs.end(), ctx->imgs.begin(), ctx->imgs.end());
@@ -2146,8 +2168,8 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
     const TangleDev& t = ctx->b_tg[k];
     for (int q = 0; q < t.n; ++q) {
       tid[t.pt_base + q] = k;
-      fwd[t.pt_base + q] = t.pt_base + fwd_l[k][q];
-      rev[t.pt_base + q] = t.pt_base + rev_l[k][q];
+      fwd[t.pt_base + q] = fwd_l[k][q] >= 0 ? t.pt_base + fwd_l[k][q] : -1;   // -1: a chain end
+      rev[t.pt_base + q] = rev_l[k][q] >= 0 ? t.pt_base + rev_l[k][q] : -1;
     }
     for (int j = t.rec_base / TILE; j < (t.rec_base + ((2 * t.n + TILE - 1) / TILE) * TILE) / TILE; ++j)
       tile_tangle[j] = k;
@@ -2156,6 +2178,14 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
   if ((rc = grow_int(ctx, &ctx->b_tid, &ctx->b_cap_tid, ntot)) ||
       (rc = grow_int(ctx, &ctx->b_tile_tangle, &ctx->b_cap_rec, rec_tot / TILE + 1)))
     return rc;
+  if (ctx->b_pins) {
+    std::vector<int> pin(ntot, -1);
+    for (int k = 0; k < K; ++k)
+      for (int q = 0; q < (int)pin_l[k].size(); ++q) pin[ctx->b_tg[k].pt_base + q] = pin_l[k][q];
+    if ((rc = grow_int(ctx, &ctx->b_pin, &ctx->b_cap_pin, ntot))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->b_pin, pin.data(), sizeof(int) * ntot, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
   if (n_tgt > ctx->b_cap_tgt) {
     long c = std::max<long>(n_tgt, 2 * ctx->b_cap_tgt);
     if ((rc = dalloc(ctx, (void**)&ctx->b_tgt_tangle, sizeof(int) * c)) ||

@@ -926,6 +926,9 @@ struct LaneTarget {
   bool valid;
   int adjA, adjB;
 };
+// Source tile of the record that closes a target's adjacent segment; -1
+// where the target has no such segment (a chain end) or is not a point.
+__device__ __forceinline__ int adj_tile(int rec) { return rec >= 0 ? rec / TILE : -1; }
 
 // Work items [w0, w1) of one target tile: item w = (source tile J_base +
 // w / n_img, image w % n_img), walked with an incremental (tile, image)
@@ -1036,8 +1039,8 @@ __device__ __forceinline__ void direct_body(const DirectArgs& A) {
   L.s = T * TILE + lane;
   L.valid = L.s < A.n;
   load3(A.P, A.cap, L.valid ? L.s : T * TILE, L.r);
-  L.adjA = L.valid ? A.adj[2 * L.s] / TILE : -1;
-  L.adjB = L.valid ? A.adj[2 * L.s + 1] / TILE : -1;
+  L.adjA = L.valid ? adj_tile(A.adj[2 * L.s]) : -1;
+  L.adjB = L.valid ? adj_tile(A.adj[2 * L.s + 1]) : -1;
   const TgtTile tt = A.tgt[T];
   double acc[3] = {0, 0, 0};
   __shared__ FarLds lds[MIXED ? 4 : 1];

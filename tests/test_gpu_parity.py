@@ -167,9 +167,9 @@ def test_errors(ctx):
         ctx.velocities(bad, 0.0, p)
     assert e.value.code == OV_E_INVALID
     bad = t.copy()
-    bad.status[3] = (T.PINNED, T.Z_L)    # walls: not in the ensemble path
+    bad.status[3] = (T.PINNED, T.Z_L)    # walls: not in the adjoint
     with pytest.raises(OVError) as e:
-        ctx.rk4_run_batch([bad], 0.0, 1e-5, 1, p)
+        ctx.velocity_vjp(bad, 0.0, p, np.ones((t.N, 3)))
     assert e.value.code == OV_E_UNSUPPORTED
     bad = t.copy()
     bad.connections[5, 0] = -1           # a free point must have both links

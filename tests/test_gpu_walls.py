@@ -242,8 +242,27 @@ def test_wall_octree_matches_reference(ctx, theta):
 
 def test_wall_unsupported_paths(ctx):
     t = wall_tangle()
-    for fn in (lambda: ctx.rk4_run_batch([copy(t)], 0.0, 1e-5, 1, params()),
-               lambda: ctx.reconnection_candidates(copy(t), 1e-3, 0.087)):
-        with pytest.raises(OVError) as e:
-            fn()
-        assert e.value.code == OV_E_UNSUPPORTED
+    with pytest.raises(OVError) as e:
+        ctx.reconnection_candidates(copy(t), 1e-3, 0.087)
+    assert e.value.code == OV_E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
+def test_wall_batch_matches_single(ctx, prec):
+    """An ensemble of tangles with PINNED and PINNED_SLIP lines next to a
+    free tangle in one batch: each member equals its single-tangle run
+    (3 steps with the periodic wrap)."""
+    members = [wall_tangle(seed=3, status=T.PINNED), wall_tangle(seed=4, status=T.PINNED_SLIP, rings=False),
+               T.random_rings(1005, 0.1, n_rings=2)]
+    p = params(precision=prec)
+    batch = [copy(m) for m in members]
+    ctx.rk4_run_batch(batch, 0.0, 1e-5, 3, [p] * len(batch), enforce_boundaries=True)
+    for k, m in enumerate(members):
+        a = copy(m)
+        ctx.rk4_run(a, 0.0, 1e-5, 3, p, enforce_boundaries=True)
+        act = m.active
+        d = np.abs(a.vnodes[act] - batch[k].vnodes[act]).max()
+        moved = np.abs(a.vnodes[act] - m.vnodes[act]).max()
+        print(f"member {k}: batch vs single {d:.3g} (moved {moved:.3g})", flush=True)
+        assert d < 1e-15 and moved > 1e-9
+        assert rel_err(batch[k].vels[act], a.vels[act])[0] < 1e-11
