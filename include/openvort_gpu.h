@@ -274,7 +274,9 @@ int ov_gpu_rk4_run_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t0, do
  * the candidates in order with the neighbour test, the "recalculate" flags,
  * the velocity test and do_reconnection gives the reference's pass.  Points
  * must lie within half a box length of a periodic box (enforce_boundaries
- * keeps them inside). */
+ * keeps them inside).  Points pinned on walls are skipped as the pair loop
+ * skips them; the reference's wall reconnections before that loop
+ * (vortex_dynamics.c:152-208) stay with the host. */
 int ov_gpu_reconnection_candidates(ov_ctx *ctx, const ov_tangle_state *tangle, double rec_dist,
                                    double rec_angle, int *pairs, int max_pairs, int *n_pairs);
 

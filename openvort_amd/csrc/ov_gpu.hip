@@ -440,7 +440,7 @@ int prepare(ov_ctx* ctx, const ov_tangle_state* t, const ov_params* p, bool allo
       if (!allow_pins)
         return fail(ctx, OV_E_UNSUPPORTED, "pinned point %d: walls are supported by ov_gpu_velocities, "
                     "ov_gpu_rk4_step/run, ov_gpu_rk4_run_batch, ov_gpu_update_tangents_normals, the "
-                    "topology edits and ov_gpu_run only", k);
+                    "topology edits, ov_gpu_run and ov_gpu_reconnection_candidates only", k);
       if (t->status[k].pin_wall < 0 || t->status[k].pin_wall > 5)
         return fail(ctx, OV_E_INVALID, "pinned point %d has pin_wall %d", k, t->status[k].pin_wall);
       pins = true;
@@ -2497,11 +2497,18 @@ int ov_gpu_reconnection_candidates(ov_ctx* ctx, const ov_tangle_state* t, double
   HIPCHK(hipSetDevice(ctx->device));
   ov_params p;
   ov_params_default(&p);
-  int rc = prepare(ctx, t, &p);
+  int rc = prepare(ctx, t, &p, true);
   if (rc) return rc;
   const int n = ctx->n;
   if (n < 2) return OV_OK;
   if (!t->tangents) return fail(ctx, OV_E_INVALID, "tangents required");
+  if (ctx->has_pins) {
+    if (ctx->pin_cap < ctx->cap) {
+      if ((rc = dalloc(ctx, (void**)&ctx->pin, sizeof(int) * ctx->cap))) return rc;
+      ctx->pin_cap = ctx->cap;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->pin, ctx->pin_s.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  }
   ctx->profile = false;
   const long cap = ctx->cap;
   if ((rc = rec_buffers(ctx, n)) || (rc = upload_vec3(ctx, t->vnodes, ctx->X0)) ||
@@ -2538,6 +2545,7 @@ int ov_gpu_reconnection_candidates(ov_ctx* ctx, const ov_tangle_state* t, double
   A.act = ctx->r_act;
   A.keys = ctx->r_keys_sorted;
   A.order = ctx->r_order;
+  A.pin = ctx->has_pins ? ctx->pin : nullptr;
   A.cap = cap;
   A.n = n;
   A.g = g;

@@ -4,8 +4,10 @@
 // The reference walks every pair k < l of points (caller indices) and, for
 // those that survive its tests in order, rewires the first acceptable l of
 // each k.  The tests that cannot change during the pass are found here:
-//   * both points free (EMPTY points are not in the stream; pinned points are
-//     rejected before any kernel runs);
+//   * both points free (EMPTY points are not in the stream; points pinned on
+//     walls, which the reference's pair loop skips, are skipped by their pin
+//     codes; its wall reconnections, vortex_dynamics.c:152-208, stay with
+//     the host);
 //   * minimum-image distance |v_k - pwrap(v_k, v_l)| <= rec_dist
 //     (vortex_dynamics.c:236-242);
 //   * |t_k| >= 0.5 and |t_l| >= 0.5 (vortex_dynamics.c:256-257; tangents are
@@ -116,6 +118,7 @@ struct RecArgs {
   const int* act;         // stream index -> caller index
   const uint64_t* keys;   // sorted cell keys
   const int* order;       // stream index of each sorted key
+  const int* pin;         // pin codes (stream order), nullptr without pinned points
   long cap;
   int n;
   CellGrid g;
@@ -131,6 +134,7 @@ __global__ void __launch_bounds__(256) kr_pairs(RecArgs A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   const int s = A.order[i];
+  if (A.pin && A.pin[s] >= 0) return;
   const int ks = A.act[s];
   double a[3], ta[3];
   load3(A.P, A.cap, s, a);
@@ -165,7 +169,7 @@ __global__ void __launch_bounds__(256) kr_pairs(RecArgs A) {
         for (; j < A.n && A.keys[j] == key; ++j) {
           const int sl = A.order[j];
           const int kl = A.act[sl];
-          if (kl <= ks) continue;
+          if (kl <= ks || (A.pin && A.pin[sl] >= 0)) continue;
           double b[3], tb[3];
           load3(A.P, A.cap, sl, b);
           load3(A.T, A.cap, sl, tb);

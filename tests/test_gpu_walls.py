@@ -243,8 +243,39 @@ def test_wall_octree_matches_reference(ctx, theta):
 def test_wall_unsupported_paths(ctx):
     t = wall_tangle()
     with pytest.raises(OVError) as e:
-        ctx.reconnection_candidates(copy(t), 1e-3, 0.087)
+        ctx.velocity_vjp(copy(t), 0.0, params(), np.ones((t.N, 3)))
     assert e.value.code == OV_E_UNSUPPORTED
+
+
+def test_wall_reconnection_candidates(ctx, oracle):
+    """A ring crossing a line pinned on the walls and a ring passing 8e-4 cm
+    above the line's pinned head: the candidates skip the pinned points (as
+    the reference's pair loop does) and equal the oracle's pair loop; driven
+    through the reference's own dynamic tests and do_reconnection they give
+    its reconnect pass (no free point lies within rec_dist / 2 of a wall, so
+    its wall reconnections do nothing)."""
+    rng = np.random.default_rng(31)
+    b = T.TangleBuilder()
+    line = wall_line(rng, 48, 0.05, 0.05, amp=0.0)
+    b.add_chain(line, (T.PINNED, T.Z_L), (T.PINNED, T.Z_H))
+    b.add_ring((0.05 + 0.006, 0.05, line[24, 2]), (0.0, 0.0, 1.0), 0.006, 60)
+    b.add_ring((0.05, 0.05 + 0.004, 0.0008), (0.0, 0.0, 1.0), 0.004, 20)
+    t = padded(b.build((0, 0, 0), (L, L, L), WALLS, IMAGES, holes=10, seed=2))
+    R = Reference()
+    p = params(hyperfriction=0)
+    R.set_params(p)
+    R.update_tangents_normals(t)
+    g = ctx.reconnection_candidates(copy(t), 1e-3, 0.087)
+    o = oracle.reconnection_candidates(copy(t), 1e-3, 0.087)
+    pinned = set(np.nonzero(t.status[:, 0] != T.FREE)[0]) - set(np.nonzero(t.status[:, 0] == T.EMPTY)[0])
+    print("candidates", len(g), "pinned points", sorted(pinned), flush=True)
+    assert len(g) > 0 and np.array_equal(g, o)
+    assert not (set(g.reshape(-1)) & pinned)
+    a, c = copy(t), copy(t)
+    na = R.reconnect(a, 0.0, p, 1e-3, 0.087)
+    nc = R.reconnect_candidates(c, 0.0, p, ctx.reconnection_candidates(copy(t), 1e-3, 0.087))
+    print("reconnections", na, nc, flush=True)
+    assert na == nc and np.array_equal(a.connections, c.connections)
 
 
 @pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
