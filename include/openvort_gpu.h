@@ -152,7 +152,7 @@ typedef struct {
 enum {
   OV_OK = 0,
   OV_E_INVALID = -1,      /* malformed tangle or arguments */
-  OV_E_UNSUPPORTED = -2,  /* a feature outside an entry point's scope (pinned points off the direct path, ...) */
+  OV_E_UNSUPPORTED = -2,  /* a feature outside an entry point's scope (pinned points in the adjoint, ...) */
   OV_E_NONFINITE = -3,    /* a result is NaN/Inf (outputs still written) */
   OV_E_DEVICE = -4,       /* HIP runtime failure */
   OV_E_NOMEM = -5         /* device or host allocation failed */

@@ -153,13 +153,14 @@ def test_wall_region_eliminations_unsupported(ctx):
     assert e.value.code == OV_E_UNSUPPORTED
 
 
-@pytest.mark.parametrize("status,prec", [(T.PINNED, T.PREC_FP64), (T.PINNED_SLIP, T.PREC_FP64),
-                                         (T.PINNED, T.PREC_MIXED)])
-def test_wall_run_matches_host_sequence(ctx, status, prec):
+@pytest.mark.parametrize("status,prec,bh", [(T.PINNED, T.PREC_FP64, 0), (T.PINNED_SLIP, T.PREC_FP64, 0),
+                                            (T.PINNED, T.PREC_MIXED, 0), (T.PINNED, T.PREC_FP64, 1)])
+def test_wall_run_matches_host_sequence(ctx, status, prec, bh):
     """ov_gpu_run over 4 steps equals the same steps driven from the host
     with the single-call entry points bit for bit: the device rebuilds the
-    stream (chains from their heads) and the pin codes as prepare() does."""
-    p = params(dl_max=DL_MAX, precision=prec, near_radius=0.01)
+    stream (chains from their heads), the pin codes as prepare() does and,
+    with the octree (bh), the tree's insertion order."""
+    p = params(dl_max=DL_MAX, precision=prec, near_radius=0.01, use_BH=bh, BH_resolution=0.5)
     tp = T.default_topology(dl_min=DL_MIN, dl_max=DL_MAX)
     t = remesh_wall_tangle(seed=21, status=status, small=[("ring", 4), ("arc", 5)])
     a, b = copy(t), copy(t)
