@@ -609,11 +609,12 @@ __device__ inline void oct_multipole(const double com[3], const OctMom& nd, cons
   acc[2] += v[2];
 }
 
-// FP32 form of oct_multipole: R = r - centre in FP64, everything after in
+// FP32 form of oct_multipole: R = r - centre in FP64 rounded to FP32 (given
+// as its negation dn, which the opening test formed), everything after in
 // FP32 (the node's moments rounded once).
-__device__ inline void oct_multipole32(const double com[3], const OctMom32& nd, const double r[3],
-                                       bool dipole, double acc[3]) {
-  const float R[3] = {(float)(r[0] - com[0]), (float)(r[1] - com[1]), (float)(r[2] - com[2])};
+__device__ inline void oct_multipole32(const float dn[3], const OctMom32& nd, bool dipole,
+                                       double acc[3]) {
+  const float R[3] = {-dn[0], -dn[1], -dn[2]};
   const float R2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
   const float ir = __builtin_amdgcn_rsqf(R2);
   const float i3 = ir * ir * ir;
@@ -684,13 +685,17 @@ __device__ __forceinline__ OctFast oct_fast_consts(const BoxDev& b, float lane_z
   F.ceps = 5e-7f + lane_zero;
   return F;
 }
-// 1: accept, 0: open, -1: within the band (the FP64 test decides)
+// 1: accept, 0: open, -1: within the band (the FP64 test decides).  dn:
+// c - rs rounded to FP32 before the wrap; its negation is rs - c rounded to
+// FP32 (the FP64 difference and the rounding are both symmetric under
+// negation), the FP32 multipole's R, which so costs no FP64 operation.
 __device__ __forceinline__ int oct_accept32(const double rs[3], const double c[3], float q,
-                                            const OctFast& F) {
+                                            const OctFast& F, float dn[3]) {
   float e[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float d = (float)(c[a] - rs[a]);
+    dn[a] = d;
     float k = d > F.h[a] ? F.nL[a] : 0.f;
     k = d < F.nh[a] ? F.L[a] : k;
     e[a] = d + k;
@@ -806,7 +811,8 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
         }
         next = after;
       } else {
-        int dec = oct_accept32(rs, nd.com, nd.q, F);
+        float dn[3];
+        int dec = oct_accept32(rs, nd.com, nd.q, F, dn);
         if (__builtin_expect(dec < 0, 0)) {
           double dv[3];
           oct_min_image(rs, nd.com, A.op.box, dv);
@@ -818,7 +824,7 @@ __device__ __forceinline__ void octree_body(const OctArgs& A) {
             // the compiler): they arrive in VGPRs, so the multipole's FP32
             // instructions dual-issue, where SGPR operands would not
             const OctMom32 mm = reinterpret_cast<const OctMom32*>(A.nmom)[node + lane_zero_i];
-            oct_multipole32(nd.com, mm, rs, dipole, acc);
+            oct_multipole32(dn, mm, dipole, acc);
           } else {
             const OctMom mm = A.nmom[node];
             oct_multipole(nd.com, mm, rs, dipole, acc);
