@@ -262,6 +262,14 @@ int ov_gpu_rk4_run_vjp(ov_ctx *ctx, const ov_tangle_state *tangle, double t0, do
                        int nsteps, const ov_params *params, int flags, int checkpoint_every,
                        const ov_vec3d *cot_final, ov_vec3d *grad_initial);
 
+/* Device memory the adjoint's segment pass may hold for its per-target-tile
+ * partial sums (6 N ceil(N / 64) doubles): within the limit each (segment,
+ * target) pair is evaluated once for both of its uses; beyond it (or with
+ * bytes = 0) in two passes that need 6 N x (a few hundred) doubles.  Default
+ * 128 MiB (tangles up to about 13,000 points).  No counterpart in the
+ * reference. */
+void ov_ctx_set_adjoint_partial_limit(ov_ctx *ctx, size_t bytes);
+
 /* Reconnection candidates of reconnect (vortex_dynamics.c:140-315): every
  * pair k < l (caller indices) of free points whose minimum-image distance is
  * at most rec_dist and whose tangents (as the tangle holds them) both have

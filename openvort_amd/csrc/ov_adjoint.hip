@@ -514,20 +514,98 @@ __global__ void __launch_bounds__(256) ka_bs_sources(AdjArgs A) {
   }
 }
 
-// red[q][p] (9 planes, q = blockIdx.y): the chunk sums of the target pass
-// (q = 0..2) and of the segment pass (q = 3..5: start, 6..8: end), in chunk
-// order.  red reuses the local pass's gnb planes, free once ka_gather ran.
-__global__ void __launch_bounds__(256) ka_bs_reduce(const double* __restrict__ tpart,
-                                                    const double* __restrict__ spart, int n_chunks,
+// Both passes in one (tangles of few target tiles): targets per lane with
+// the target pass's sums, and the terms of each (segment, image) pair for
+// the segment's ends, (dB/da)^T gu and (dB/db)^T gu, summed over the
+// wavefront's 64 targets by a butterfly that halves the values each lane
+// carries (6 -> 3 -> 2 -> 1 over lane distances 32, 16, 8, then a full sum
+// over 4, 2, 1), written per target tile: spart[T][6][m].  Each pair is
+// evaluated once instead of once per pass.
+__global__ void __launch_bounds__(256) ka_bs_fused(AdjArgs A, double* __restrict__ spart) {
+  const int lane = threadIdx.x & 63;
+  const int T = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (T >= A.n_tiles) return;
+  const int chunk = blockIdx.y;
+  const int m0 = (int)(((long)A.n * chunk) / A.n_chunks);
+  const int m1 = (int)(((long)A.n * (chunk + 1)) / A.n_chunks);
+  const int k = T * TILE + lane;
+  const bool valid = k < A.n;
+  const int kk = valid ? k : T * TILE;
+  const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0;
+  const int vi = (h4 ? 2 : 0) + (h3 ? 1 : 0);   // which of this half's values the lane ends with
+  double r[3], w[3], g[3] = {0, 0, 0};
+  load3(A.P, A.cap, kk, r);
+  load3(A.gu, A.cap, kk, w);
+  for (int m = m0; m < m1; ++m) {
+    double a[3], b[3];
+    load3(A.P, A.cap, m, a);
+    load3(A.SB, A.cap, m, b);
+    const int fm = A.fwd[m];
+    const bool adjacent = m == kk || fm == kk;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int I = 0; I < A.n_img; ++I) {
+      const ImgDev im = A.img[I];
+      if (im.primary && adjacent) continue;
+      double rs[3], wi[3], gR[3], gRp[3];
+      image_map(im, r, w, rs, wi);
+      if (!pair_vjp(a, b, rs, wi, gR, gRp)) continue;
+      double gr[3] = {-(gR[0] + gRp[0]), -(gR[1] + gRp[1]), -(gR[2] + gRp[2])};
+      if (im.axis >= 0) gr[im.axis] = -gr[im.axis];
+      g[0] += gr[0];
+      g[1] += gr[1];
+      g[2] += gr[2];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        v[i] += gR[i];
+        v[3 + i] += gRp[i];
+      }
+    }
+    if (!valid) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) v[i] = 0;
+    }
+    double u[4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double send = h5 ? v[i] : v[3 + i], keep = h5 ? v[3 + i] : v[i];
+      u[i] = keep + __shfl_xor(send, 32, 64);
+    }
+    u[3] = 0;
+    double x2[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double send = h4 ? u[j] : u[2 + j], keep = h4 ? u[2 + j] : u[j];
+      x2[j] = keep + __shfl_xor(send, 16, 64);
+    }
+    double x;
+    {
+      const double send = h3 ? x2[0] : x2[1], keep = h3 ? x2[1] : x2[0];
+      x = keep + __shfl_xor(send, 8, 64);
+    }
+    x += __shfl_xor(x, 4, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    if ((lane & 7) == 0 && vi < 3) spart[((long)T * 6 + (h5 ? 3 : 0) + vi) * A.cap + m] = x;
+  }
+  if (valid) store3(A.part + (long)chunk * 3 * A.cap, A.cap, k, g);
+}
+
+// red[q][p] (9 planes, q = blockIdx.y): the sums of the target pass's t_terms
+// chunks (q = 0..2) and of the segment pass's s_terms chunks or target tiles
+// (q = 3..5: start, 6..8: end), in order.  red reuses the local pass's gnb
+// planes, free once ka_gather ran.
+__global__ void __launch_bounds__(256) ka_bs_reduce(const double* __restrict__ tpart, int t_terms,
+                                                    const double* __restrict__ spart, int s_terms,
                                                     int n, long cap, double* __restrict__ red) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (p >= n) return;
   const double* src = q < 3 ? tpart + (long)q * cap + p : spart + (long)(q - 3) * cap + p;
   const long stride = (q < 3 ? 3 : 6) * cap;
+  const int terms = q < 3 ? t_terms : s_terms;
   double a = 0;
 #pragma unroll 8
-  for (int c = 0; c < n_chunks; ++c) a += src[c * stride];
+  for (int c = 0; c < terms; ++c) a += src[c * stride];
   red[(long)q * cap + p] = a;
 }
 

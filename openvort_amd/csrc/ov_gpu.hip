@@ -153,6 +153,7 @@ struct ov_ctx {
          *a_G = nullptr, *a_Gtmp = nullptr, *a_gK[4] = {nullptr, nullptr, nullptr, nullptr};
   long a_cap = 0;
   double *a_tpart = nullptr, *a_spart = nullptr, *a_ck = nullptr, *a_store = nullptr;
+  size_t a_fused_limit = (size_t)128 << 20;   // per-target-tile partials of the fused segment pass
   size_t a_tpart_bytes = 0, a_spart_bytes = 0, a_ck_bytes = 0, a_store_bytes = 0;
 
   // reconnection candidates
@@ -1335,8 +1336,11 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
   // about four waves per SIMD of 256 CUs, at least 64 segments (targets) per chunk
   int chunks = (4096 + n_tiles - 1) / n_tiles;
   chunks = std::max(1, std::min(chunks, std::min(256, (n + 63) / 64)));
+  // tangles of few target tiles: one fused pass, segment terms per target tile
+  const bool fused = (size_t)6 * cap * n_tiles * sizeof(double) <= ctx->a_fused_limit;
+  const int s_terms = fused ? n_tiles : chunks;
   if ((rc = grow_bytes(ctx, &ctx->a_tpart, &ctx->a_tpart_bytes, sizeof(double) * 3 * cap * chunks)) ||
-      (rc = grow_bytes(ctx, &ctx->a_spart, &ctx->a_spart_bytes, sizeof(double) * 6 * cap * chunks)))
+      (rc = grow_bytes(ctx, &ctx->a_spart, &ctx->a_spart_bytes, sizeof(double) * 6 * cap * s_terms)))
     return rc;
   AdjArgs A;
   A.P = X;
@@ -1351,11 +1355,15 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
   A.n_chunks = chunks;
   const dim3 grid((unsigned)((n_tiles + 3) / 4), (unsigned)chunks);
   A.part = ctx->a_tpart;
-  if ((rc = launch(ctx, "ka_bs_targets", ka_bs_targets, grid, dim3(256), A))) return rc;
-  A.part = ctx->a_spart;
-  if ((rc = launch(ctx, "ka_bs_sources", ka_bs_sources, grid, dim3(256), A))) return rc;
+  if (fused) {
+    if ((rc = launch(ctx, "ka_bs_fused", ka_bs_fused, grid, dim3(256), A, ctx->a_spart))) return rc;
+  } else {
+    if ((rc = launch(ctx, "ka_bs_targets", ka_bs_targets, grid, dim3(256), A))) return rc;
+    A.part = ctx->a_spart;
+    if ((rc = launch(ctx, "ka_bs_sources", ka_bs_sources, grid, dim3(256), A))) return rc;
+  }
   if ((rc = launch(ctx, "ka_bs_reduce", ka_bs_reduce, dim3(grid1(n).x, 9), dim3(256),
-                   (const double*)ctx->a_tpart, (const double*)ctx->a_spart, chunks, n, cap,
+                   (const double*)ctx->a_tpart, chunks, (const double*)ctx->a_spart, s_terms, n, cap,
                    ctx->a_gnb)))
     return rc;
   return launch(ctx, "ka_bs_finish", ka_bs_finish, grid1(n), dim3(256), (const double*)ctx->a_gl,
@@ -2337,6 +2345,10 @@ int ov_gpu_rk4_run_batch(ov_ctx* ctx, ov_tangle_state* const* tangles, int ntang
   ctx->n = ntot;
   if (flag) return fail(ctx, OV_E_NONFINITE, "non-finite %s", (flag & 2) ? "position" : "velocity");
   return OV_OK;
+}
+
+void ov_ctx_set_adjoint_partial_limit(ov_ctx* ctx, size_t bytes) {
+  if (ctx) ctx->a_fused_limit = bytes;
 }
 
 void ov_ctx_batch_tuning(const ov_ctx* ctx, int* unit, int* blocks, double* tune_ms) {

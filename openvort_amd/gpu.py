@@ -36,6 +36,7 @@ SYMBOLS = (
     "ov_ctx_tree_stats",
     "ov_gpu_rk4_run_batch",
     "ov_ctx_batch_tuning",
+    "ov_ctx_set_adjoint_partial_limit",
     "ov_gpu_velocity_vjp",
     "ov_gpu_rk4_run_vjp",
     "ov_gpu_reconnection_candidates",
@@ -99,6 +100,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.ov_gpu_rk4_run_batch.restype = C.c_int
     lib.ov_ctx_batch_tuning.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_double)]
     lib.ov_ctx_batch_tuning.restype = None
+    lib.ov_ctx_set_adjoint_partial_limit.argtypes = [C.c_void_p, C.c_size_t]
+    lib.ov_ctx_set_adjoint_partial_limit.restype = None
     V3 = C.c_void_p   # ov_vec3d* as a numpy (N, 3) float64 buffer
     lib.ov_gpu_velocity_vjp.argtypes = [C.c_void_p, P(TangleState), C.c_double, P(Params), V3, V3, C.c_int]
     lib.ov_gpu_velocity_vjp.restype = C.c_int
@@ -273,6 +276,10 @@ class Context:
         if rc in (OV_OK, OV_E_NONFINITE):
             t.next_free = ts.next_free
         return self._check(rc, allow_nonfinite)
+
+    def set_adjoint_partial_limit(self, nbytes: int) -> None:
+        """Memory for the adjoint's fused segment pass (0: two passes)."""
+        self.lib.ov_ctx_set_adjoint_partial_limit(self.ptr, int(nbytes))
 
     def batch_tuning(self):
         u, b, ms = C.c_int(), C.c_int(), C.c_double()

@@ -61,6 +61,28 @@ def test_velocity_vjp_many_tiles_matches_autograd(ctx):
     assert e < AD_TOL
 
 
+@pytest.mark.parametrize("name", ["ring_mf_flows", "periodic_holes", "mirror", "small_loops"])
+def test_segment_pass_forms_agree(ctx, name):
+    """The fused segment pass (each pair once, target-tile partials) and the
+    two-pass form (ov_ctx_set_adjoint_partial_limit 0) give the same VJPs to
+    rounding (1e-13; they sum the segment terms in different orders), and the
+    two-pass form matches autograd as the fused one does."""
+    t, p = AC.case(name)
+    w = AC.seeded((t.N, 3), 15)
+    g1 = ctx.velocity_vjp(t, 0.0, p, w)
+    r1 = ctx.rk4_run_vjp(t, 0.0, 2e-6, 3, p, w, checkpoint_every=2)
+    ctx.set_adjoint_partial_limit(0)
+    try:
+        g2 = ctx.velocity_vjp(t, 0.0, p, w)
+        r2 = ctx.rk4_run_vjp(t, 0.0, 2e-6, 3, p, w, checkpoint_every=2)
+    finally:
+        ctx.set_adjoint_partial_limit(128 << 20)
+    e, er = _err(g2, g1), _err(r2, r1)
+    ea = _err(g2[t.active], _ref_velocity_vjp(t, p, w[t.active]))
+    print(name, "two-pass vs fused", e, er, "two-pass vs autograd", ea, flush=True)
+    assert e < 1e-13 and er < 1e-13 and ea < AD_TOL
+
+
 @pytest.mark.parametrize("name", ["ring_mf_flows", "periodic_holes", "mirror"])
 def test_velocity_vjp_matches_oracle_fd(ctx, oracle, name):
     t, p = AC.case(name)
