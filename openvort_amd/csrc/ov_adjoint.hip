@@ -271,6 +271,9 @@ __device__ void local_velocity(const D3 p[5][3], const D3 u[3], const BoxDev& bo
 
 // Local VJP of every point k: gnb[j][k] = (dvel_k/dp_j)^T w_k for the five
 // stencil slots j (k-2, k-1, k, k+1, k+2), gu[k] = (dvel_k/du_k)^T w_k.
+// One thread per (point, seeding): blockIdx.y is the seeding (0..4 the
+// stencil slots, 5 the Biot-Savart sum), so the six seedings of a point run
+// side by side rather than in one thread's sequence.
 __global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
                                                 const int* __restrict__ fwd,
                                                 const int* __restrict__ rev,
@@ -279,6 +282,7 @@ __global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
                                                 BoxDev box, PhysDev ph, double t,
                                                 double* __restrict__ gnb, double* __restrict__ gu) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int seed = blockIdx.y;
   if (k >= n) return;
   int ids[5];
   ids[2] = k;
@@ -290,8 +294,8 @@ __global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
   load3(w, cap, k, wk);
   if (have_bs) load3(bs, cap, k, uk);
   // The imposed flows depend on the point's own position only: evaluate them
-  // and their 3x3 Jacobians once, seeded along x, y, z of point k, and hand
-  // the derivative part only to the seeding of point k (slot 2).
+  // and their 3x3 Jacobians seeded along x, y, z of point k, and hand the
+  // derivative part only to the seeding of point k (slot 2).
   D3 fs[3], fn[3];
   {
     D3 r[3];
@@ -305,7 +309,7 @@ __global__ void __launch_bounds__(128) ka_local(const double* __restrict__ P,
     else
       fn[0] = fn[1] = fn[2] = dc(0);
   }
-  for (int seed = 0; seed < 6; ++seed) {
+  {
     D3 p[5][3], u[3];
     for (int j = 0; j < 5; ++j)
       for (int i = 0; i < 3; ++i) {

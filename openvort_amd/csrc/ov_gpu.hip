@@ -1319,7 +1319,7 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
   const long cap = ctx->cap;
   const int have_bs = ctx->lia_only ? 0 : 1;
   int rc;
-  if ((rc = launch(ctx, "ka_local", ka_local, dim3((unsigned)((n + 127) / 128)), dim3(128), X,
+  if ((rc = launch(ctx, "ka_local", ka_local, dim3((unsigned)((n + 127) / 128), 6), dim3(128), X,
                    (const int*)ctx->fwd, (const int*)ctx->rev, bs, have_bs, W, n, cap, ctx->box,
                    ctx->ph, t, ctx->a_gnb, ctx->a_gu)))
     return rc;
