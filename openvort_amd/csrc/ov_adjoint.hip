@@ -433,11 +433,12 @@ struct AdjArgs {
   const double* gu;   // cotangent of each point's Biot-Savart sum (without kappa/4pi)
   const ImgDev* img;
   long cap;
+  long ps;            // plane stride of the partial sums (the point count rounded up to 64)
   int n;
   int n_tiles;
   int n_img;
   int n_chunks;
-  double* part;       // targets: [chunks][3][cap]; segments: [chunks][6][cap]
+  double* part;       // targets: [chunks][3][ps]; segments: [chunks][6][ps]
 };
 
 // Targets per lane, (segment, image) pairs wave-uniform: partial sums of
@@ -474,7 +475,7 @@ __global__ void __launch_bounds__(256) ka_bs_targets(AdjArgs A) {
       g[2] += gr[2];
     }
   }
-  if (valid) store3(A.part + (long)chunk * 3 * A.cap, A.cap, k, g);
+  if (valid) store3(A.part + (long)chunk * 3 * A.ps, A.ps, k, g);
 }
 
 // Segments per lane, (target, image) pairs wave-uniform: partial sums of
@@ -512,9 +513,9 @@ __global__ void __launch_bounds__(256) ka_bs_sources(AdjArgs A) {
     }
   }
   if (valid) {
-    double* out = A.part + (long)chunk * 6 * A.cap;
-    store3(out, A.cap, m, ga);
-    store3(out + 3 * A.cap, A.cap, m, gb);
+    double* out = A.part + (long)chunk * 6 * A.ps;
+    store3(out, A.ps, m, ga);
+    store3(out + 3 * A.ps, A.ps, m, gb);
   }
 }
 
@@ -523,7 +524,7 @@ __global__ void __launch_bounds__(256) ka_bs_sources(AdjArgs A) {
 // the segment's ends, (dB/da)^T gu and (dB/db)^T gu, summed over the
 // wavefront's 64 targets by a butterfly that halves the values each lane
 // carries (6 -> 3 -> 2 -> 1 over lane distances 32, 16, 8, then a full sum
-// over 4, 2, 1), written per target tile: spart[T][6][m].  Each pair is
+// over 4, 2, 1), written per target tile: spart[T][6][ps].  Each pair is
 // evaluated once instead of once per pass.
 __global__ void __launch_bounds__(256) ka_bs_fused(AdjArgs A, double* __restrict__ spart) {
   const int lane = threadIdx.x & 63;
@@ -589,9 +590,9 @@ __global__ void __launch_bounds__(256) ka_bs_fused(AdjArgs A, double* __restrict
     x += __shfl_xor(x, 4, 64);
     x += __shfl_xor(x, 2, 64);
     x += __shfl_xor(x, 1, 64);
-    if ((lane & 7) == 0 && vi < 3) spart[((long)T * 6 + (h5 ? 3 : 0) + vi) * A.cap + m] = x;
+    if ((lane & 7) == 0 && vi < 3) spart[((long)T * 6 + (h5 ? 3 : 0) + vi) * A.ps + m] = x;
   }
-  if (valid) store3(A.part + (long)chunk * 3 * A.cap, A.cap, k, g);
+  if (valid) store3(A.part + (long)chunk * 3 * A.ps, A.ps, k, g);
 }
 
 // red[q][p] (9 planes, q = blockIdx.y): the sums of the target pass's t_terms
@@ -600,12 +601,12 @@ __global__ void __launch_bounds__(256) ka_bs_fused(AdjArgs A, double* __restrict
 // planes, free once ka_gather ran.
 __global__ void __launch_bounds__(256) ka_bs_reduce(const double* __restrict__ tpart, int t_terms,
                                                     const double* __restrict__ spart, int s_terms,
-                                                    int n, long cap, double* __restrict__ red) {
+                                                    int n, long ps, long cap, double* __restrict__ red) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (p >= n) return;
-  const double* src = q < 3 ? tpart + (long)q * cap + p : spart + (long)(q - 3) * cap + p;
-  const long stride = (q < 3 ? 3 : 6) * cap;
+  const double* src = q < 3 ? tpart + (long)q * ps + p : spart + (long)(q - 3) * ps + p;
+  const long stride = (q < 3 ? 3 : 6) * ps;
   const int terms = q < 3 ? t_terms : s_terms;
   double a = 0;
 #pragma unroll 8

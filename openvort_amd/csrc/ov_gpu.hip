@@ -1336,11 +1336,13 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
   // about four waves per SIMD of 256 CUs, at least 64 segments (targets) per chunk
   int chunks = (4096 + n_tiles - 1) / n_tiles;
   chunks = std::max(1, std::min(chunks, std::min(256, (n + 63) / 64)));
+  // partial planes of n points (rounded up to 64), whatever the buffers' capacity;
   // tangles of few target tiles: one fused pass, segment terms per target tile
-  const bool fused = (size_t)6 * cap * n_tiles * sizeof(double) <= ctx->a_fused_limit;
+  const long ps = (long)n_tiles * TILE;
+  const bool fused = (size_t)6 * ps * n_tiles * sizeof(double) <= ctx->a_fused_limit;
   const int s_terms = fused ? n_tiles : chunks;
-  if ((rc = grow_bytes(ctx, &ctx->a_tpart, &ctx->a_tpart_bytes, sizeof(double) * 3 * cap * chunks)) ||
-      (rc = grow_bytes(ctx, &ctx->a_spart, &ctx->a_spart_bytes, sizeof(double) * 6 * cap * s_terms)))
+  if ((rc = grow_bytes(ctx, &ctx->a_tpart, &ctx->a_tpart_bytes, sizeof(double) * 3 * ps * chunks)) ||
+      (rc = grow_bytes(ctx, &ctx->a_spart, &ctx->a_spart_bytes, sizeof(double) * 6 * ps * s_terms)))
     return rc;
   AdjArgs A;
   A.P = X;
@@ -1349,6 +1351,7 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
   A.gu = ctx->a_gu;
   A.img = ctx->img;
   A.cap = cap;
+  A.ps = ps;
   A.n = n;
   A.n_tiles = n_tiles;
   A.n_img = (int)ctx->imgs.size();
@@ -1363,8 +1366,8 @@ int velocity_vjp_dev(ov_ctx* ctx, const double* X, const double* bs, const doubl
     if ((rc = launch(ctx, "ka_bs_sources", ka_bs_sources, grid, dim3(256), A))) return rc;
   }
   if ((rc = launch(ctx, "ka_bs_reduce", ka_bs_reduce, dim3(grid1(n).x, 9), dim3(256),
-                   (const double*)ctx->a_tpart, chunks, (const double*)ctx->a_spart, s_terms, n, cap,
-                   ctx->a_gnb)))
+                   (const double*)ctx->a_tpart, chunks, (const double*)ctx->a_spart, s_terms, n, ps,
+                   cap, ctx->a_gnb)))
     return rc;
   return launch(ctx, "ka_bs_finish", ka_bs_finish, grid1(n), dim3(256), (const double*)ctx->a_gl,
                 (const double*)ctx->a_gnb, (const int*)ctx->rev, n, cap,

@@ -83,6 +83,24 @@ def test_segment_pass_forms_agree(ctx, name):
     assert e < 1e-13 and er < 1e-13 and ea < AD_TOL
 
 
+def test_fused_pass_after_a_large_tangle(ctx):
+    """The fused pass's partial planes are strided by the tangle's own point
+    count, so a context that held a large tangle before (buffers of its
+    capacity) still takes the fused pass for a small one, with the same
+    result as a fresh context's."""
+    t, p = AC.case("ring_mf_flows")
+    w = AC.seeded((t.N, 3), 16)
+    g0 = ctx.velocity_vjp(t, 0.0, p, w)
+    big = T.random_rings(1009, 0.1, n_points_target=60000, spacing=2e-4, r_lo=0.002, r_hi=0.02)
+    ctx.velocities(big, 0.0, T.default_params(vortex_width=1.3e-8, hyperfriction=0))
+    ctx.reset_stats()
+    g1 = ctx.velocity_vjp(t, 0.0, p, w, profile=True)
+    n_fused, _ = ctx.kernel_stats("ka_bs_fused")
+    print("after", len(big.active), "points: fused launches", n_fused, flush=True)
+    assert n_fused == 1
+    assert np.array_equal(g1, g0)
+
+
 @pytest.mark.parametrize("name", ["ring_mf_flows", "periodic_holes", "mirror"])
 def test_velocity_vjp_matches_oracle_fd(ctx, oracle, name):
     t, p = AC.case(name)
