@@ -1029,11 +1029,11 @@ int evaluate(ov_ctx* ctx, const double* X, bool geom, double* Tg, double* Ng, do
   return OV_OK;
 }
 
-// Copy of one vec3 plane set (3 planes of cap doubles) on the context stream.
+// Copy of one vec3 plane set (the n points of 3 planes of stride cap) on the
+// context stream.
 int copy_planes(ov_ctx* ctx, double* dst, const double* src) {
-  HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * 3 * (size_t)ctx->cap, hipMemcpyDeviceToDevice,
-                        ctx->stream));
-  return OV_OK;
+  if (ctx->n == 0) return OV_OK;
+  return launch(ctx, "k_copy3", k_copy3, grid1(ctx->n), dim3(256), src, dst, ctx->n, ctx->cap);
 }
 
 // update_tangle(t) + rk4_step(t, dt) on the device-resident state in X0;

@@ -1184,6 +1184,18 @@ __global__ void __launch_bounds__(256) k_epilogue(const double* __restrict__ P,
 // RK4 stages (reference vortex_dynamics.c:36-52, 58-117)
 // ---------------------------------------------------------------------------
 
+// dst = src for the first n entries of three planes of stride cap (a kernel,
+// so that captured graphs hold kernels only and copy what is used, whatever
+// the buffers' capacity).
+__global__ void __launch_bounds__(256) k_copy3(const double* __restrict__ src, double* __restrict__ dst,
+                                               int n, long cap) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  dst[s] = src[s];
+  dst[cap + s] = src[cap + s];
+  dst[2 * cap + s] = src[2 * cap + s];
+}
+
 // Xs = X0 + h K (euler_step2: move = v dt, then x + move, each rounded as the
 // reference rounds it: no fused multiply-add)
 __global__ void __launch_bounds__(256) k_stage(const double* __restrict__ X0,
