@@ -129,8 +129,9 @@ def adjoint_section(ctx, T):
         ctx.rk4_run_vjp(base, 0.0, dt, nsteps, p, np.ones((k.npts, 3)), checkpoint_every=ck, profile=True)
         prof_ms = ctx.last_device_ms()
         kstats = {}
-        for kn in ("ka_bs_fused", "ka_bs_targets", "ka_bs_sources", "ka_bs_reduce", "ka_local", "ka_gather", "ka_bs_finish",
-                   "k_direct_fp64_small", "k_direct_mixed_small", "k_geometry", "k_epilogue"):
+        for kn in ("ka_bs_fused", "ka_bs_targets", "ka_bs_sources", "ka_bs_reduce", "ka_local",
+                   "ka_gather", "ka_bs_finish", "k_direct_fp64_small", "k_direct_mixed_small",
+                   "k_geometry", "k_epilogue"):
             nl, ms = ctx.kernel_stats(kn)
             if nl:
                 kstats[kn] = {"launches": nl, "total_ms": round(ms, 2)}
