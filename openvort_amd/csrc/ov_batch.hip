@@ -97,13 +97,8 @@ __global__ void __launch_bounds__(256) kb_wall_geometry(const double* __restrict
     store3(T, cap, s, tv);
     store3(Nn, cap, s, nm);
   }
-  // lia_velocity (tangle.c:530-551) with the walked neighbours
-  const double lf = chord(p0, pn, t.box), lr = chord(pp, p0, t.box);
-  const double pi = 3.14159265358979323846;
-  const double beta =
-      t.ph.kappa * log(2 * sqrt(lf * lr) / sqrt(sqrt(2.718281828459045)) / t.ph.a0) / 4 / pi;
-  const double v[3] = {(tv[1] * nm[2] - tv[2] * nm[1]) * beta, (tv[2] * nm[0] - tv[0] * nm[2]) * beta,
-                       (tv[0] * nm[1] - tv[1] * nm[0]) * beta};
+  double v[3];
+  wall_lia_point(p0, pn, pp, tv, nm, t.box, t.ph, v);
   store3(lia, cap, s, v);
 }
 

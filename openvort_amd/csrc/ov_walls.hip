@@ -216,6 +216,19 @@ __device__ inline bool wall_tn_point(int s, bool herm, const double p0[3], const
   return true;
 }
 
+// lia_velocity (tangle.c:530-551) of point p0 with its walked neighbours pn,
+// pp and its s', s''.
+__device__ inline void wall_lia_point(const double p0[3], const double pn[3], const double pp[3],
+                                      const double t[3], const double nm[3], const BoxDev& box,
+                                      const PhysDev& ph, double v[3]) {
+  const double lf = chord(p0, pn, box), lr = chord(pp, p0, box);
+  const double pi = 3.14159265358979323846;
+  const double beta = ph.kappa * log(2 * sqrt(lf * lr) / sqrt(sqrt(2.718281828459045)) / ph.a0) / 4 / pi;
+  v[0] = (t[1] * nm[2] - t[2] * nm[1]) * beta;
+  v[1] = (t[2] * nm[0] - t[0] * nm[2]) * beta;
+  v[2] = (t[0] * nm[1] - t[1] * nm[0]) * beta;
+}
+
 __global__ void __launch_bounds__(256) k_wall_dxi(const double* __restrict__ P, const int* __restrict__ fwd,
                                                   const int* __restrict__ rev, const int* __restrict__ pin,
                                                   int n, long cap, BoxDev box, double* __restrict__ dxi,
@@ -263,12 +276,8 @@ __global__ void __launch_bounds__(256) k_wall_geometry(const double* __restrict_
     store3(T, cap, s, t);
     store3(Nn, cap, s, nm);
   }
-  // lia_velocity (tangle.c:530-551) with the walked neighbours
-  const double lf = chord(p0, pn, box), lr = chord(pp, p0, box);
-  const double pi = 3.14159265358979323846;
-  const double beta = ph.kappa * log(2 * sqrt(lf * lr) / sqrt(sqrt(2.718281828459045)) / ph.a0) / 4 / pi;
-  const double v[3] = {(t[1] * nm[2] - t[2] * nm[1]) * beta, (t[2] * nm[0] - t[0] * nm[2]) * beta,
-                       (t[0] * nm[1] - t[1] * nm[0]) * beta};
+  double v[3];
+  wall_lia_point(p0, pn, pp, t, nm, box, ph, v);
   store3(lia, cap, s, v);
 }
 
